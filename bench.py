@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: randomized-Cholesky factor + PCG to relative residual 1e-6 on one
+MI355X, with the CPU oracle timed beside it.
+
+One step = rc_factor (elimination + assembly of the resident system) +
+rc_solve_resident (PCG to tol with the RHS already in HBM).  Both calls return
+after synchronising the library's stream; the timed loop is additionally
+bracketed by hipDeviceSynchronize.
+
+Default workload: 3D 7-point Poisson SDDM 128^3 with
+diagonal = degree + 1e-2, converted to a Laplacian with an appended ground
+vertex, nnz-sorted ordering (seed 1), FP64 factor and PCG, one N(0,1) RHS
+(seed 42), tol 1e-6.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2]
+"""
+import argparse
+import ctypes
+import importlib
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "parallel-randomized-cholesky_amd")
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_package():
+    spec = importlib.util.spec_from_file_location("rchol_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rchol_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod, importlib.import_module("rchol_amd.workloads")
+
+
+def device_sync():
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+CONFIGS = {
+    "c1": dict(workload="grid2d_64x64_unit_nnzsort", dims=(64, 64), kind="laplacian", weights="unit", prec=64,
+               rhs="normal_projected"),
+    "c2": dict(workload="sddm3d_128_ground_nnzsort", dims=(128, 128, 128), kind="sddm", prec=64, rhs="normal"),
+    "c3": dict(workload="grid3d_256_loguniform_nnzsort_f32factor", dims=(256, 256, 256), kind="laplacian",
+               weights="loguniform", prec=32, rhs="normal_projected"),
+    "c3_f64": dict(workload="grid3d_256_loguniform_nnzsort_f64", dims=(256, 256, 256), kind="laplacian",
+                   weights="loguniform", prec=64, rhs="normal_projected"),
+}
+
+
+def make_system(wl, cfg):
+    if cfg["kind"] == "sddm":
+        return wl.grid_sddm(cfg["dims"], 1e-2)
+    return wl.grid_laplacian(cfg["dims"], cfg.get("weights", "unit"), seed=7)
+
+
+def cpu_baseline(wl, cfg, b_full, tol, max_n):
+    """The oracle (single-threaded C port of the reference algorithm) on the
+    same workload, or on the same generator at a reduced size when the full
+    size exceeds max_n rows (time scaled linearly in n; stated in `sample`)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    dims = cfg["dims"]
+    full_n = int(np.prod(dims))
+    scale = 1.0
+    sdims = dims
+    while int(np.prod(sdims)) > max_n:
+        sdims = tuple(max(2, d // 2) for d in sdims)
+    sc = dict(cfg, dims=sdims)
+    n, r, c, v = make_system(wl, sc)
+    kind = "sddm" if cfg["kind"] == "sddm" else "laplacian"
+    fac, sys_, perm, appended = orc.prepare_system(n, r, c, v, kind=kind, order="nnz-sort", order_seed=1)
+    if sdims == dims:
+        b = b_full
+    else:
+        b = np.array([orc.normal(42, i) for i in range(n)])
+        if kind != "sddm":
+            b -= b.mean()
+        scale = full_n / n
+    t0 = time.perf_counter()
+    f = orc.factor(fac, seed=1)
+    if appended:
+        f = orc.factor_trim_last(f)
+    t1 = time.perf_counter()
+    x, st = orc.pcg(sys_, f, b[perm], tol=tol, precision=cfg["prec"])
+    t2 = time.perf_counter()
+    sample = (f"oracle (1 thread) factor+PCG on {'x'.join(map(str, sdims))} "
+              f"({n} rows): factor {t1 - t0:.2f}s + PCG {t2 - t1:.2f}s, {st['iterations']} iterations")
+    if scale != 1.0:
+        sample += f"; scaled x{scale:.1f} to {full_n} rows"
+    return {"value": (t2 - t0) * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
+            "iterations": st["iterations"], "factor_s": t1 - t0, "pcg_s": t2 - t1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", type=int, default=0, help="override factor precision (32/64)")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-max-n", type=int, default=2_200_000)
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    rc, wl = load_package()
+    cfg = dict(CONFIGS[args.config])
+    if args.precision:
+        cfg["prec"] = args.precision
+    log(f"[bench] config {args.config}: generating {cfg['workload']}")
+    t = time.perf_counter()
+    n, r, c, v = make_system(wl, cfg)
+    log(f"[bench] generated n={n} triplets={len(r)} in {time.perf_counter() - t:.1f}s")
+
+    s = rc.Solver(device=0, seed=1, precision=cfg["prec"])
+    t = time.perf_counter()
+    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if cfg["kind"] == "sddm" else rc.INPUT_LAPLACIAN)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    b = s.make_rhs(rc.RHS_NORMAL if cfg["rhs"] == "normal" else rc.RHS_NORMAL_PROJECTED, 42)
+    log(f"[bench] host input path (Laplacian, ordering, RHS) {time.perf_counter() - t:.1f}s")
+
+    # warmup: first factor also prepares and uploads the system
+    fst = sst = None
+    for w in range(max(args.warmup, 1)):
+        t = time.perf_counter()
+        fst = s.factor()
+        s.set_rhs(b)
+        sst = s.solve_resident(args.tol)
+        log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f}s iters={sst['iterations']} "
+            f"factor={fst['factor_ms']:.1f}ms assemble={fst['assemble_ms']:.1f}ms pcg={sst['pcg_ms']:.1f}ms")
+
+    device_sync()
+    times, fms, ams, pms, iters = [], [], [], [], []
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        fst = s.factor()
+        sst = s.solve_resident(args.tol)
+        times.append(time.perf_counter() - t)
+        fms.append(fst["factor_ms"]); ams.append(fst["assemble_ms"]); pms.append(sst["pcg_ms"])
+        iters.append(sst["iterations"])
+        log(f"[bench] step {k}: {times[-1]:.3f}s iters={sst['iterations']} rel_res={sst['rel_res_true']:.2e}")
+    device_sync()
+    wall = time.perf_counter() - t_all
+
+    # per-kernel timing on the library stream (hipEvents), algorithmic bytes
+    kern = {}
+    for name, kid in (("spmv", rc.KERNEL_SPMV), ("trsv_fwd", rc.KERNEL_TRSV_FWD), ("trsv_bwd", rc.KERNEL_TRSV_BWD)):
+        ms, by = s.time_kernel(kid, reps=args.kernel_reps)
+        kern[name] = {"ms": ms, "bytes": by, "GBps": by / ms / 1e6}
+        log(f"[bench] kernel {name}: {ms:.4f} ms  {by / 1e6:.1f} MB  {by / ms / 1e6:.1f} GB/s")
+    fac_gbs = fst["bytes_factor"] / fst["factor_ms"] / 1e6
+    kern["factor"] = {"ms": fst["factor_ms"], "bytes": fst["bytes_factor"], "GBps": fac_gbs}
+    per_solve = {k: kern[k]["ms"] * (np.mean(iters) + 1) for k in ("spmv", "trsv_fwd", "trsv_bwd")}
+    per_solve["factor"] = fst["factor_ms"]
+    dom = max(per_solve, key=per_solve.get)
+    kname = {"spmv": "k_spmv_plain", "trsv_fwd": "k_trsv_fwd", "trsv_bwd": "k_trsv_bwd", "factor": "k_factor"}[dom]
+    achieved = kern[dom]["GBps"]
+
+    x = s.get_solution()
+    res = {
+        "metric": "factor+PCG time to rel-res 1e-6 (s)",
+        "value": float(np.mean(times)),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(times) * 1e3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if cfg["prec"] == 64 else "f32-factor/f64-pcg",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n": int(s.size()[0]), "nnz": int(s.size()[1]),
+                   "factor_precision": cfg["prec"], "order": "nnz-sort", "tol": args.tol,
+                   "stop_rule": "reference sqrt(r'z/r0'z0)"},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "detail": {
+            "wall_s_timed_loop": wall,
+            "factor_ms": float(np.mean(fms)), "assemble_ms": float(np.mean(ams)), "pcg_ms": float(np.mean(pms)),
+            "iterations": iters, "rel_res_true": sst["rel_res_true"], "nnz_factor": fst["nnz_factor"],
+            "fills": fst["fills"], "nlevels": fst["nlevels"], "max_degree": fst["max_degree"],
+            "kernels": kern,
+        },
+    }
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        res["cpu_baseline"] = cpu_baseline(wl, cfg, b, args.tol, args.cpu_max_n)
+        log(f"[bench] cpu: {res['cpu_baseline']['sample']}")
+    else:
+        res["cpu_baseline"] = None
+    s.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

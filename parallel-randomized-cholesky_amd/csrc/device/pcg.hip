@@ -1,0 +1,423 @@
+// pcg.hip -- PCG with the randomized-Cholesky preconditioner on gfx950.
+//
+// Follows gpu_implementation/solver.hpp:149-233 (z = L^-T D+ L^-1 r, D+
+// divides only where D != 0) and solver.hpp:457-569 (x0 = 0; alpha =
+// r'z / p'Ap; x += alpha p; r -= alpha Ap; z = M r; beta = r'z_new / r'z;
+// p = z + beta p; stop on sqrt(r'z / r0'z0) <= tol or k = maxit).
+//
+// One iteration is four kernels with device-resident scalars and no host
+// synchronisation, so a run of iterations is captured in one hipGraph:
+//   K1 spmv   p_new = z + beta p_old (fused into the SpMV gather), Ap = A p_new,
+//             block partials of p'Ap, last block computes alpha;
+//   K2 fwd    r -= alpha Ap and x += alpha p on the row being solved, then the
+//             forward sweep y = L^-1 r over level-ordered chunks;
+//   K3 bwd    z = L^-T (D+ y) over chunks in reverse level order;
+//   K4 dot    r'z block partials, last block computes beta and the stop test.
+// The sweeps are persistent: a wave claims the next 64-row chunk of the level
+// order; a row reads each dependency with a plain (cached) load and, only if
+// it still holds the sentinel, polls it with agent-scope loads.  Values are
+// written once per sweep with agent-scope stores, so no fences are needed;
+// each consumer resets the vector it drains back to the sentinel.
+// All reductions use a fixed grid and a fixed summation order.
+#include "common.cuh"
+#include "kernels.h"
+
+namespace rc {
+
+namespace {
+
+constexpr int kRedBlock = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int wid = threadIdx.x / kWave;
+    if (lane_id() == 0) lds[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += lds[w];
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
+// Sum of partials[0..m) by one block in a fixed order; valid in thread 0.
+__device__ __forceinline__ double reduce_partials(const double *partials, int m, double *lds)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) v += ld_relaxed_val(partials + i);
+    return block_sum(v, lds);
+}
+
+// Returns true in every thread of the block that arrived last.
+__device__ __forceinline__ bool last_block(unsigned int *arrive, int *lds_flag)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        fence_release();
+        unsigned int old = atomicAdd(arrive, 1u);
+        *lds_flag = (old == gridDim.x - 1) ? 1 : 0;
+        if (*lds_flag) fence_acquire();
+    }
+    __syncthreads();
+    return *lds_flag != 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T wait_value(T *p, long long spin_ticks, PcgScalars *sc)
+{
+    T v = *p;                                   // cached fast path
+    if (!is_sent(v)) return v;
+    long long t0 = wall_clock64();
+    while (true) {
+        v = ld_relaxed_val(p);
+        if (!is_sent(v)) return v;
+        if (ld_relaxed(&sc->status) & kPcgTimeout) return (T)0;
+        if (wall_clock64() - t0 > spin_ticks) {
+            atomicOr(&sc->status, kPcgTimeout);
+            return (T)0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// ---------------------------------------------------------------- K1
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p_old, double *p_new, int first)
+{
+    __shared__ double lds[kRedBlock / kWave];
+    __shared__ int flag;
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    const double beta = first ? 0.0 : sc->beta;
+    const T *z = (const T *)a.z;
+    double acc_dot = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double pi = beta * p_old[i] + (double)z[i];
+        double acc = 0.0;
+        for (int e = a.a_ptr[i]; e < a.a_ptr[i + 1]; ++e) {
+            int j = a.a_ind[e];
+            double pj = beta * p_old[j] + (double)z[j];
+            acc = fma_t(a.a_val[e], pj, acc);
+        }
+        p_new[i] = pi;
+        a.ap[i] = acc;
+        acc_dot += pi * acc;
+    }
+    double bs = block_sum(acc_dot, lds);
+    if (threadIdx.x == 0) st_relaxed_val(a.partials + blockIdx.x, bs);
+    if (last_block(&sc->arrive_spmv, &flag)) {
+        double pap = reduce_partials(a.partials, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            sc->pap = pap;
+            if (!(pap > 0.0) || !isfinite(pap)) {      // breakdown (solver.hpp has no check)
+                sc->status |= kPcgBreakdown;
+                sc->done = 1;
+                sc->alpha = 0.0;
+            } else {
+                sc->alpha = sc->rz / pap;
+            }
+            sc->arrive_spmv = 0u;
+        }
+    }
+}
+
+// Plain SpMV for timing: y = A x (same gather pattern as K1 with beta = 0).
+__global__ void __launch_bounds__(kRedBlock) k_spmv_plain(int n, const int *ptr, const int *ind, const double *val,
+                                                          const double *x, double *y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) acc = fma_t(val[e], x[ind[e]], acc);
+        y[i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------- K2
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_trsv_fwd(SolveArgs a, const double *p_cur, int update_rx)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const double alpha = update_rx ? sc->alpha : 0.0;
+    const T *lval = (const T *)a.l_val;
+    T *y = (T *)a.y;
+    T *z = (T *)a.z;
+    const int lane = lane_id();
+    while (true) {
+        int c = 0;
+        if (lane == 0) c = (int)atomicAdd(a.ctr_fwd, 1u);
+        c = bcast(c, 0);
+        if (c >= a.nchunks) break;
+        const int cnt = a.chunk_cnt[c];
+        if (lane < cnt) {
+            const int k = a.order[a.chunk_start[c] + lane];
+            double rk = a.r[k];
+            if (update_rx) {
+                rk = fma_t(-alpha, a.ap[k], rk);
+                a.r[k] = rk;
+                a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
+            }
+            T acc = (T)rk;
+            for (int e = a.l_ptr[k]; e < a.l_ptr[k + 1]; ++e) {
+                T v = wait_value(y + a.l_ind[e], a.spin_ticks, sc);
+                acc = fma_t(-lval[e], v, acc);
+            }
+            st_relaxed_val(y + k, unsent(acc));
+            z[k] = sent_value(T());
+        }
+        if (ld_relaxed(&sc->status) & kPcgTimeout) break;
+    }
+}
+
+// ---------------------------------------------------------------- K3
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_trsv_bwd(SolveArgs a)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const T *ltval = (const T *)a.lt_val;
+    const T *diag = (const T *)a.diag;
+    T *y = (T *)a.y;
+    T *z = (T *)a.z;
+    const int lane = lane_id();
+    while (true) {
+        int c = 0;
+        if (lane == 0) c = (int)atomicAdd(a.ctr_bwd, 1u);
+        c = bcast(c, 0);
+        if (c >= a.nchunks) break;
+        c = a.nchunks - 1 - c;
+        const int cnt = a.chunk_cnt[c];
+        if (lane < cnt) {
+            const int k = a.order[a.chunk_start[c] + lane];
+            T yk = y[k];
+            y[k] = sent_value(T());
+            T dk = diag[k];
+            if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
+            T acc = yk;
+            for (int e = a.lt_ptr[k]; e < a.lt_ptr[k + 1]; ++e) {
+                T v = wait_value(z + a.lt_ind[e], a.spin_ticks, sc);
+                acc = fma_t(-ltval[e], v, acc);
+            }
+            st_relaxed_val(z + k, unsent(acc));
+        }
+        if (ld_relaxed(&sc->status) & kPcgTimeout) break;
+    }
+}
+
+// ---------------------------------------------------------------- K4
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_dot(SolveArgs a, int init)
+{
+    __shared__ double lds[kRedBlock / kWave];
+    __shared__ int flag;
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const T *z = (const T *)a.z;
+    const bool need_rr = sc->stop_rule == 1;
+    double acc = 0.0, acc_rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double ri = a.r[i];
+        acc += ri * (double)z[i];
+        if (need_rr) acc_rr += ri * ri;
+    }
+    double bs = block_sum(acc, lds);
+    double bs_rr = need_rr ? block_sum(acc_rr, lds) : 0.0;
+    if (threadIdx.x == 0) {
+        st_relaxed_val(a.partials + blockIdx.x, bs);
+        st_relaxed_val(a.partials + gridDim.x + blockIdx.x, bs_rr);
+    }
+    if (last_block(&sc->arrive_dot, &flag)) {
+        double rz = reduce_partials(a.partials, gridDim.x, lds);
+        double rr = need_rr ? reduce_partials(a.partials + gridDim.x, gridDim.x, lds) : 0.0;
+        if (threadIdx.x == 0) {
+            bool go;
+            if (init) {
+                sc->rz0 = rz;
+                sc->rz = rz;
+                sc->beta = 0.0;
+                sc->iter = 0;
+                go = need_rr ? (sc->bb > 0.0 && sqrt(rr) / sqrt(sc->bb) > sc->tol)
+                             : (rz > sc->tol);                         // solver.hpp:493
+            } else {
+                sc->beta = rz / sc->rz;
+                sc->rz = rz;
+                sc->iter += 1;
+                go = need_rr ? (sqrt(rr) / sqrt(sc->bb) > sc->tol)
+                             : (sqrt(rz) / sqrt(sc->rz0) > sc->tol);  // solver.hpp:536
+            }
+            sc->rr = rr;
+            if (!go || sc->iter >= sc->maxit || sc->status != 0) sc->done = 1;
+            sc->arrive_dot = 0u;
+        }
+    }
+}
+
+// ||A x - b||^2 and ||b||^2 (solver.hpp:549-569).
+__global__ void __launch_bounds__(kRedBlock) k_residual(SolveArgs a)
+{
+    __shared__ double lds[kRedBlock / kWave];
+    __shared__ int flag;
+    PcgScalars *sc = a.sc;
+    double acc = 0.0, accb = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int e = a.a_ptr[i]; e < a.a_ptr[i + 1]; ++e) s = fma_t(a.a_val[e], a.x[a.a_ind[e]], s);
+        double d = s - a.b[i];
+        acc += d * d;
+        accb += a.b[i] * a.b[i];
+    }
+    double bs = block_sum(acc, lds);
+    double bb = block_sum(accb, lds);
+    if (threadIdx.x == 0) {
+        st_relaxed_val(a.partials + blockIdx.x, bs);
+        st_relaxed_val(a.partials + gridDim.x + blockIdx.x, bb);
+    }
+    if (last_block(&sc->arrive_res, &flag)) {
+        double r2 = reduce_partials(a.partials, gridDim.x, lds);
+        double b2 = reduce_partials(a.partials + gridDim.x, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            sc->res2 = r2;
+            sc->bnorm2 = b2;
+            sc->arrive_res = 0u;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_fill_sentinel(T *v, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = sent_value(T());
+}
+
+__global__ void k_pcg_prepare(SolveArgs a)
+{
+    // r = b, x = 0, p = 0, Ap = 0 (solver.hpp:268-277, 462-466)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        a.r[i] = a.b[i];
+        a.x[i] = 0.0;
+        a.p0[i] = 0.0;
+        a.p1[i] = 0.0;
+        a.ap[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+}
+
+__global__ void k_bnorm(SolveArgs a)
+{
+    __shared__ double lds[kRedBlock / kWave];
+    __shared__ int flag;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) acc += a.b[i] * a.b[i];
+    double bs = block_sum(acc, lds);
+    if (threadIdx.x == 0) st_relaxed_val(a.partials + blockIdx.x, bs);
+    if (last_block(&a.sc->arrive_res, &flag)) {
+        double b2 = reduce_partials(a.partials, gridDim.x, lds);
+        if (threadIdx.x == 0) { a.sc->bb = b2; a.sc->arrive_res = 0u; }
+    }
+}
+
+__global__ void k_gather(const double *src, const int *idx, double *dst, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[idx[i]];
+}
+__global__ void k_scatter(const double *src, const int *idx, double *dst, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[idx[i]] = src[i];
+}
+__global__ void k_reset_counters(unsigned int *a, unsigned int *b)
+{
+    if (threadIdx.x == 0) { *a = 0u; *b = 0u; }
+}
+
+int sweep_blocks(const SolveArgs &a)
+{
+    int g = a.nchunks < 16384 ? a.nchunks : 16384;
+    return g > 0 ? g : 1;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_fill_sentinel(T *v, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, v, n);
+}
+
+template <typename T>
+void launch_pcg_init(const SolveArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.y, a.n);
+    hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, a.n);
+    hipLaunchKernelGGL(k_pcg_prepare, dim3(2048), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_bnorm, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)a.p0, 0);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 1);
+}
+
+// parity 0: p0 -> p1, parity 1: p1 -> p0.  The first loop iteration uses
+// parity 0 with beta forced to zero, which makes p = z (solver.hpp:477).
+template <typename T>
+void launch_pcg_iteration(const SolveArgs &a, int parity, hipStream_t s)
+{
+    double *p_old = parity ? a.p1 : a.p0;
+    double *p_new = parity ? a.p0 : a.p1;
+    hipLaunchKernelGGL(k_spmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_old, p_new, 0);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)p_new, 1);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 0);
+}
+
+template <typename T>
+void launch_spmv_only(const SolveArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spmv_plain, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a.n, a.a_ptr, a.a_ind, a.a_val,
+                       (const double *)a.p0, a.ap);
+}
+
+template <typename T>
+void launch_trsv_fwd_only(const SolveArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)a.p0, 0);
+}
+
+template <typename T>
+void launch_trsv_bwd_only(const SolveArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+}
+
+void launch_residual(const SolveArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
+}
+
+void launch_gather(const double *src, const int *idx, double *dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
+}
+void launch_scatter(const double *src, const int *idx, double *dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scatter, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
+}
+void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, a, b);
+}
+
+template void launch_fill_sentinel<float>(float *, int, hipStream_t);
+template void launch_fill_sentinel<double>(double *, int, hipStream_t);
+template void launch_pcg_init<float>(const SolveArgs &, hipStream_t);
+template void launch_pcg_init<double>(const SolveArgs &, hipStream_t);
+template void launch_pcg_iteration<float>(const SolveArgs &, int, hipStream_t);
+template void launch_pcg_iteration<double>(const SolveArgs &, int, hipStream_t);
+template void launch_spmv_only<float>(const SolveArgs &, hipStream_t);
+template void launch_spmv_only<double>(const SolveArgs &, hipStream_t);
+template void launch_trsv_fwd_only<float>(const SolveArgs &, hipStream_t);
+template void launch_trsv_fwd_only<double>(const SolveArgs &, hipStream_t);
+template void launch_trsv_bwd_only<float>(const SolveArgs &, hipStream_t);
+template void launch_trsv_bwd_only<double>(const SolveArgs &, hipStream_t);
+
+}  // namespace rc

@@ -1,0 +1,1235 @@
+// capi.cpp -- implementation of the rchol.h C ABI: one handle = one HIP
+// stream + the device memory of one system.  Host work is limited to the
+// input boundary (Matrix Market, Laplacian build, ordering, ground append);
+// the factorization and the PCG run in the HIP kernels of csrc/device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/rchol.h"
+#include "../device/kernels.h"
+#include "mmio.hpp"
+#include "philox.h"
+#include "sparse.hpp"
+#include "gnn_host.hpp"
+
+using namespace rc;
+
+namespace {
+
+struct DevVec {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevVec() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    // grow-only; returns hipSuccess or the allocation error
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        if (b == 0) b = 8;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) { p = nullptr; return e; }
+        bytes = b;
+        return hipSuccess;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+int64_t pow2_at_least(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct rc_handle_s {
+    rc_options opt{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    long long spin_ticks = 0;
+    int num_blocks = 0;
+    bool host_only = false;
+
+    // loaded system
+    int kind = -1;
+    Csc base;
+    int n_user = 0;
+    std::vector<int> perm;          // perm[new] = old (user numbering)
+    std::vector<float> gnn_w;
+
+    // prepared system
+    bool host_ready = false;
+    bool prepared = false;
+    Csc fac;                        // permuted Laplacian that is factored
+    Csc sys;                        // permuted PCG system
+    bool trim = false;
+    int nf = 0, ns = 0;
+    int64_t nnz_low = 0;
+
+    // factor inputs / working set
+    DevVec lo_ptr, lo_row, lo_w;
+    DevVec dep, head, nfill, queue, level, ctr, pool, out_row, out_val, out_start, out_cnt, diag64, scratch;
+    long long pool_cap = 0, out_cap = 0;
+    int scratch_cap = 0;
+
+    // factor result (solve numbering)
+    bool factored = false;
+    int prec = 64;
+    DevVec lt_ptr, lt_ind, lt_val, l_ptr, l_ind, l_val, diagT, order, chunk_start, chunk_cnt;
+    DevVec tmp_a, tmp_b, tmp_scan;
+    int64_t nnz_lt = 0;
+    int nlevels = 0, nchunks = 0;
+    rc_factor_stats fstats{};
+
+    // PCG
+    DevVec a_ptr, a_ind, a_val;
+    DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d;
+    int reduce_blocks = 1024;
+    bool rhs_set = false;
+    std::vector<double> xref;
+    hipGraphExec_t graph = nullptr;
+    int graph_prec = 0;
+
+    ~rc_handle_s()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+#define RC_HIP(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+            return e_ == hipErrorOutOfMemory ? RC_ERR_NOMEM : RC_ERR_HIP;                      \
+        }                                                                                      \
+    } while (0)
+
+#define RC_CHECK_HANDLE()                      \
+    do {                                       \
+        if (!h) return RC_ERR_ARG;             \
+    } while (0)
+
+static int fail(rc_handle_t h, int code, const std::string &msg)
+{
+    h->err = msg;
+    return code;
+}
+
+static void invalidate_graph(rc_handle_t h)
+{
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+}
+
+extern "C" {
+
+const char *rc_status_string(int s)
+{
+    switch (s) {
+    case RC_OK: return "ok";
+    case RC_ERR_ARG: return "bad argument";
+    case RC_ERR_IO: return "i/o error";
+    case RC_ERR_FORMAT: return "malformed input";
+    case RC_ERR_NOT_DOMINANT: return "not diagonally dominant";
+    case RC_ERR_STATE: return "call out of order";
+    case RC_ERR_HIP: return "HIP runtime error";
+    case RC_ERR_POOL: return "device pool exhausted";
+    case RC_ERR_TIMEOUT: return "device wait timed out";
+    case RC_ERR_BREAKDOWN: return "PCG breakdown";
+    case RC_ERR_NO_DEVICE: return "no HIP device";
+    case RC_ERR_NOMEM: return "out of memory";
+    case RC_ERR_NOT_SYMMETRIC: return "matrix not symmetric";
+    case RC_ERR_MAXIT: return "maximum iterations reached";
+    default: return "unknown status";
+    }
+}
+
+void rc_default_options(rc_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->seed = 1;
+    o->num_blocks = 0;
+    o->factor_precision = 64;
+    o->stop_rule = RC_STOP_REFERENCE;
+    o->maxit = 300;
+    o->pool_factor = 4.0;
+    o->verbose = 0;
+}
+
+int rc_create(rc_handle_t *out, const rc_options *opt)
+{
+    if (!out) return RC_ERR_ARG;
+    *out = nullptr;
+    rc_options o;
+    rc_default_options(&o);
+    if (opt) o = *opt;
+    if (o.factor_precision != 32 && o.factor_precision != 64) return RC_ERR_ARG;
+    if (o.maxit < 0 || o.pool_factor <= 0.0) return RC_ERR_ARG;
+    if (o.device == -1) {                           // host-only handle: input, ordering, RHS
+        rc_handle_t h = new (std::nothrow) rc_handle_s();
+        if (!h) return RC_ERR_NOMEM;
+        h->opt = o;
+        h->host_only = true;
+        *out = h;
+        return RC_OK;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RC_ERR_NO_DEVICE;
+    if (o.device < 0 || o.device >= ndev) return RC_ERR_ARG;
+    rc_handle_t h = new (std::nothrow) rc_handle_s();
+    if (!h) return RC_ERR_NOMEM;
+    h->opt = o;
+    if (hipSetDevice(o.device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return RC_ERR_HIP;
+    }
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, o.device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
+    h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
+    *out = h;
+    return RC_OK;
+}
+
+int rc_destroy(rc_handle_t h)
+{
+    if (!h) return RC_ERR_ARG;
+    if (!h->host_only) (void)hipSetDevice(h->opt.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return RC_OK;
+}
+
+const char *rc_last_error(rc_handle_t h)
+{
+    if (!h) return "null handle";
+    return h->err.c_str();
+}
+
+// ------------------------------------------------------------------ input
+static void reset_system(rc_handle_t h)
+{
+    h->host_ready = false;
+    h->prepared = false;
+    h->factored = false;
+    h->rhs_set = false;
+    h->perm.clear();
+    h->xref.clear();
+    invalidate_graph(h);
+}
+
+int rc_set_triplets(rc_handle_t h, int n, int64_t m, const int *rows, const int *cols, const double *vals, int kind)
+{
+    RC_CHECK_HANDLE();
+    if (n <= 0 || m < 0 || (m > 0 && (!rows || !cols || !vals))) return fail(h, RC_ERR_ARG, "bad triplet arrays");
+    if (kind != RC_INPUT_LAPLACIAN && kind != RC_INPUT_GROUNDED && kind != RC_INPUT_SDDM)
+        return fail(h, RC_ERR_ARG, "unknown input kind");
+    reset_system(h);
+    Csc a;
+    int rc = build_laplacian(n, m, rows, cols, vals, a);
+    if (rc != RC_OK) return fail(h, rc, "triplet index out of range");
+    if (!is_symmetric(a)) return fail(h, RC_ERR_NOT_SYMMETRIC, "input matrix is not symmetric");
+    if (kind == RC_INPUT_GROUNDED && n < 2) return fail(h, RC_ERR_ARG, "grounded input needs n >= 2");
+    h->base = std::move(a);
+    h->kind = kind;
+    h->n_user = kind == RC_INPUT_GROUNDED ? n - 1 : n;
+    return RC_OK;
+}
+
+int rc_load_mtx(rc_handle_t h, const char *path, int kind)
+{
+    RC_CHECK_HANDLE();
+    if (!path) return fail(h, RC_ERR_ARG, "null path");
+    MtxTriplets t;
+    int rc = read_mtx(path, t, h->err);
+    if (rc != RC_OK) return rc;
+    if (t.nrows != t.ncols) return fail(h, RC_ERR_FORMAT, "matrix is not square");   // pre_process.hpp:358
+    return rc_set_triplets(h, t.nrows, (int64_t)t.rows.size(), t.rows.data(), t.cols.data(), t.vals.data(), kind);
+}
+
+int rc_get_size(rc_handle_t h, int64_t *n, int64_t *nnz)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (n) *n = h->n_user;
+    if (nnz) {
+        if (h->kind == RC_INPUT_GROUNDED) {
+            Csc t;
+            trim_last(h->base, t);
+            *nnz = t.nnz();
+        } else {
+            *nnz = h->base.nnz();
+        }
+    }
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------ ordering
+int rc_order(rc_handle_t h, int method, uint64_t seed)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    const int nv = h->n_user;
+    const int skip = h->kind == RC_INPUT_GROUNDED ? h->base.n - 1 : -1;
+    std::vector<int> p;
+    switch (method) {
+    case RC_ORDER_AS_IS:
+        p.resize((size_t)nv);
+        for (int i = 0; i < nv; ++i) p[(size_t)i] = i;
+        break;
+    case RC_ORDER_NNZ_SORT: p = nnz_sort_perm(h->base, nv, seed, skip); break;
+    case RC_ORDER_RANDOM: p = random_perm(nv, seed); break;
+    case RC_ORDER_AMD:
+        p = amd_perm(h->base, nv);
+        if ((int)p.size() != nv) return fail(h, RC_ERR_ARG, "AMD ordering failed");
+        break;
+    case RC_ORDER_LEARNED:
+        if (h->gnn_w.empty()) p = nnz_sort_perm(h->base, nv, seed, skip);      // fallback
+        else {
+            int rc = gnn_order(h->base, nv, skip, h->gnn_w, seed, p, h->err);
+            if (rc != RC_OK) return rc;
+        }
+        break;
+    default: return fail(h, RC_ERR_ARG, "unknown ordering");
+    }
+    h->perm = std::move(p);
+    h->host_ready = false;
+    h->prepared = false;
+    h->factored = false;
+    h->rhs_set = false;
+    invalidate_graph(h);
+    return RC_OK;
+}
+
+int rc_get_permutation(rc_handle_t h, int *perm)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0 || !perm) return fail(h, RC_ERR_ARG, "no matrix or null array");
+    for (int i = 0; i < h->n_user; ++i) perm[i] = h->perm.empty() ? i : h->perm[(size_t)i];
+    return RC_OK;
+}
+
+int rc_set_gnn_weights(rc_handle_t h, const float *w, int64_t count)
+{
+    RC_CHECK_HANDLE();
+    if (count < 0 || (count > 0 && !w)) return fail(h, RC_ERR_ARG, "bad weights");
+    if (count > 0 && count != gnn_weight_count()) return fail(h, RC_ERR_ARG, "weight count mismatch");
+    h->gnn_w.assign(w, w + count);
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------ prepare
+static int prepare_host(rc_handle_t h)
+{
+    if (h->host_ready) return RC_OK;
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (h->perm.empty()) {
+        h->perm.resize((size_t)h->n_user);
+        for (int i = 0; i < h->n_user; ++i) h->perm[(size_t)i] = i;
+    }
+    if (h->kind == RC_INPUT_LAPLACIAN) {
+        permute_sym(h->base, h->perm, h->fac);
+        h->sys = h->fac;
+        h->trim = false;
+    } else if (h->kind == RC_INPUT_GROUNDED) {
+        std::vector<int> full(h->perm);
+        full.push_back(h->base.n - 1);                 // ground stays last
+        permute_sym(h->base, full, h->fac);
+        trim_last(h->fac, h->sys);
+        h->trim = true;
+    } else {
+        Csc sp;
+        permute_sym(h->base, h->perm, sp);             // order, then append (write_graph.jl:333-383)
+        int g = ground_append(sp, h->fac);
+        if (g < 0) return fail(h, g, "SDDM input is not diagonally dominant");
+        h->trim = (g == 1);
+        h->sys = std::move(sp);
+    }
+    h->nf = h->fac.n;
+    h->ns = h->sys.n;
+    if (h->sys.nnz() > 2147483647LL || h->fac.nnz() > 2147483647LL)
+        return fail(h, RC_ERR_ARG, "nnz exceeds int32 indexing");
+    h->host_ready = true;
+    return RC_OK;
+}
+
+static int prepare(rc_handle_t h)
+{
+    if (h->prepared) return RC_OK;
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    int rc0 = prepare_host(h);
+    if (rc0 != RC_OK) return rc0;
+
+    // factor input: strict lower part
+    std::vector<int> lp, lr;
+    std::vector<double> lw;
+    int rc = lower_part(h->fac, lp, lr, lw);
+    if (rc != RC_OK) return fail(h, rc, "lower part too large");
+    h->nnz_low = (int64_t)lr.size();
+    RC_HIP(h->lo_ptr.ensure(lp.size() * sizeof(int)));
+    RC_HIP(h->lo_row.ensure(std::max<size_t>(lr.size(), 1) * sizeof(int)));
+    RC_HIP(h->lo_w.ensure(std::max<size_t>(lw.size(), 1) * sizeof(double)));
+    RC_HIP(hipMemcpyAsync(h->lo_ptr.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (!lr.empty()) {
+        RC_HIP(hipMemcpyAsync(h->lo_row.p, lr.data(), lr.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        RC_HIP(hipMemcpyAsync(h->lo_w.p, lw.data(), lw.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+
+    // PCG system (CSR == CSC), int32
+    const int n = h->ns;
+    std::vector<int> ap((size_t)n + 1);
+    for (int i = 0; i <= n; ++i) ap[(size_t)i] = (int)h->sys.ptr[(size_t)i];
+    RC_HIP(h->a_ptr.ensure(ap.size() * sizeof(int)));
+    RC_HIP(h->a_ind.ensure(std::max<size_t>(h->sys.ind.size(), 1) * sizeof(int)));
+    RC_HIP(h->a_val.ensure(std::max<size_t>(h->sys.val.size(), 1) * sizeof(double)));
+    RC_HIP(hipMemcpyAsync(h->a_ptr.p, ap.data(), ap.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (h->sys.nnz() > 0) {
+        RC_HIP(hipMemcpyAsync(h->a_ind.p, h->sys.ind.data(), h->sys.ind.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        RC_HIP(hipMemcpyAsync(h->a_val.p, h->sys.val.data(), h->sys.val.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    RC_HIP(h->perm_d.ensure((size_t)std::max(n, 1) * sizeof(int)));
+    RC_HIP(hipMemcpyAsync(h->perm_d.p, h->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    h->prepared = true;
+    h->factored = false;
+    h->rhs_set = false;
+    invalidate_graph(h);
+    return RC_OK;
+}
+
+int rc_get_prepared_csc(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *colptr, int *rowind, double *val)
+{
+    RC_CHECK_HANDLE();
+    int rc = prepare_host(h);
+    if (rc != RC_OK) return rc;
+    if (n) *n = h->fac.n;
+    if (nnz) *nnz = h->fac.nnz();
+    if (colptr) std::memcpy(colptr, h->fac.ptr.data(), h->fac.ptr.size() * sizeof(int64_t));
+    if (rowind) std::memcpy(rowind, h->fac.ind.data(), h->fac.ind.size() * sizeof(int));
+    if (val) std::memcpy(val, h->fac.val.data(), h->fac.val.size() * sizeof(double));
+    return RC_OK;
+}
+
+int rc_write_prepared_mtx(rc_handle_t h, const char *path)
+{
+    RC_CHECK_HANDLE();
+    if (!path) return fail(h, RC_ERR_ARG, "null path");
+    int rc = prepare_host(h);
+    if (rc != RC_OK) return rc;
+    return write_mtx_csc(path, h->fac.n, h->fac.n, h->fac.ptr.data(), h->fac.ind.data(), h->fac.val.data(), h->err);
+}
+
+// ------------------------------------------------------------------ factor
+static int alloc_factor_work(rc_handle_t h)
+{
+    const size_t nf = (size_t)h->nf;
+    RC_HIP(h->dep.ensure(nf * sizeof(int)));
+    RC_HIP(h->head.ensure(nf * sizeof(int)));
+    RC_HIP(h->nfill.ensure(nf * sizeof(int)));
+    RC_HIP(h->queue.ensure(nf * sizeof(int)));
+    RC_HIP(h->level.ensure(nf * sizeof(int)));
+    RC_HIP(h->out_start.ensure(nf * sizeof(long long)));
+    RC_HIP(h->out_cnt.ensure(nf * sizeof(int)));
+    RC_HIP(h->diag64.ensure(nf * sizeof(double)));
+    RC_HIP(h->ctr.ensure(kCtrCount * sizeof(unsigned long long)));
+    RC_HIP(h->pool.ensure((size_t)h->pool_cap * sizeof(FillRec)));
+    RC_HIP(h->out_row.ensure((size_t)h->out_cap * sizeof(int)));
+    RC_HIP(h->out_val.ensure((size_t)h->out_cap * sizeof(double)));
+    RC_HIP(h->scratch.ensure((size_t)h->num_blocks * (size_t)h->scratch_cap * (size_t)factor_scratch_bytes_per_entry()));
+    return RC_OK;
+}
+
+static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
+{
+    if (h->pool_cap == 0) {
+        h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low));
+        h->out_cap = h->pool_cap + h->nnz_low + 16;
+        h->scratch_cap = 2048;
+    }
+    *regrowths = 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int rc = alloc_factor_work(h);
+        if (rc != RC_OK) return rc;
+        FactorArgs a;
+        a.n = h->nf;
+        a.lo_ptr = h->lo_ptr.as<int>();
+        a.lo_row = h->lo_row.as<int>();
+        a.lo_w = h->lo_w.as<double>();
+        a.dep = h->dep.as<int>();
+        a.head = h->head.as<int>();
+        a.nfill = h->nfill.as<int>();
+        a.queue = h->queue.as<int>();
+        a.level = h->level.as<int>();
+        a.ctr = h->ctr.as<unsigned long long>();
+        a.pool = h->pool.as<FillRec>();
+        a.pool_cap = h->pool_cap;
+        a.out_row = h->out_row.as<int>();
+        a.out_val = h->out_val.as<double>();
+        a.out_cap = h->out_cap;
+        a.out_start = h->out_start.as<long long>();
+        a.out_cnt = h->out_cnt.as<int>();
+        a.diag = h->diag64.as<double>();
+        a.scratch = h->scratch.as<char>();
+        a.scratch_cap = h->scratch_cap;
+        a.seed = h->opt.seed;
+        a.spin_ticks = h->spin_ticks;
+        RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), h->stream));
+        hipEvent_t e0, e1;
+        RC_HIP(hipEventCreate(&e0));
+        RC_HIP(hipEventCreate(&e1));
+        RC_HIP(hipEventRecord(e0, h->stream));
+        launch_factor(a, h->num_blocks, h->stream);
+        RC_HIP(hipGetLastError());
+        RC_HIP(hipEventRecord(e1, h->stream));
+        unsigned long long c[kCtrCount];
+        RC_HIP(hipMemcpyAsync(c, h->ctr.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipStreamSynchronize(h->stream));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_out = ms;
+        unsigned long long st = c[kCtrStatus];
+        if (st == 0) {
+            if (c[kCtrQHead] < (unsigned long long)h->nf || c[kCtrQTail] != (unsigned long long)h->nf)
+                return fail(h, RC_ERR_STATE, "elimination did not retire every column (queue " +
+                                                 std::to_string(c[kCtrQTail]) + "/" + std::to_string(h->nf) + ")");
+            h->fstats.nnz_factor = (int64_t)c[kCtrNnzOff] + h->nf;
+            h->fstats.fills = (int64_t)c[kCtrFills];
+            h->fstats.max_degree = (int64_t)c[kCtrMaxDeg];
+            h->fstats.zero_pivots = (int32_t)c[kCtrZeroPivots];
+            return RC_OK;
+        }
+        if (st & (kStatusTimeout | kStatusInconsistent))
+            return fail(h, (st & kStatusTimeout) ? RC_ERR_TIMEOUT : RC_ERR_STATE,
+                        "elimination kernel aborted (status " + std::to_string(st) + ")");
+        if (st & kStatusFillPool) h->pool_cap *= 2;
+        if (st & (kStatusOutPool | kStatusFillPool)) h->out_cap = h->pool_cap + h->nnz_low + 16;
+        if (st & kStatusScratch) h->scratch_cap = (int)std::min<int64_t>(pow2_at_least((int64_t)c[kCtrScratchNeed]), 1 << 28);
+        ++*regrowths;
+    }
+    return fail(h, RC_ERR_POOL, "elimination pools exhausted after regrowth");
+}
+
+}  // extern "C"
+
+template <typename T>
+static int assemble(rc_handle_t h, double *ms_out)
+{
+    hipStream_t s = h->stream;
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    RC_HIP(hipEventRecord(e0, s));
+    const int ns = h->ns;
+    AssembleArgs a;
+    a.n = h->nf;
+    a.ns = ns;
+    a.trim = h->trim ? 1 : 0;
+    a.out_start = h->out_start.as<long long>();
+    a.out_cnt = h->out_cnt.as<int>();
+    a.out_row = h->out_row.as<int>();
+    a.out_val = h->out_val.as<double>();
+    a.diag_in = h->diag64.as<double>();
+    a.level = h->level.as<int>();
+
+    const size_t nsz = (size_t)ns + 1;
+    RC_HIP(h->tmp_a.ensure(nsz * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(ns + 1) * sizeof(int) + 64));
+    RC_HIP(h->lt_ptr.ensure(nsz * sizeof(int)));
+    launch_lt_counts(a, h->tmp_a.as<int>(), s);
+    scan_exclusive(h->tmp_a.as<int>(), h->lt_ptr.as<int>(), ns, h->tmp_scan.as<int>(), s);
+    int nnz = 0;
+    RC_HIP(hipMemcpyAsync(&nnz, h->lt_ptr.as<int>() + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    h->nnz_lt = nnz;
+    RC_HIP(h->lt_ind.ensure((size_t)std::max(nnz, 1) * sizeof(int)));
+    RC_HIP(h->lt_val.ensure((size_t)std::max(nnz, 1) * sizeof(T)));
+    RC_HIP(h->diagT.ensure((size_t)std::max(ns, 1) * sizeof(T)));
+    launch_lt_fill<T>(a, h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.as<T>(), h->diagT.as<T>(), s);
+
+    // transpose
+    RC_HIP(h->l_ptr.ensure(nsz * sizeof(int)));
+    RC_HIP(h->l_ind.ensure((size_t)std::max(nnz, 1) * sizeof(int)));
+    RC_HIP(h->l_val.ensure((size_t)std::max(nnz, 1) * sizeof(T)));
+    RC_HIP(h->tmp_b.ensure(nsz * sizeof(int)));
+    RC_HIP(hipMemsetAsync(h->tmp_a.p, 0, nsz * sizeof(int), s));
+    launch_count_rows(nnz, h->lt_ind.as<int>(), h->tmp_a.as<int>(), s);
+    scan_exclusive(h->tmp_a.as<int>(), h->l_ptr.as<int>(), ns, h->tmp_scan.as<int>(), s);
+    RC_HIP(hipMemcpyAsync(h->tmp_b.p, h->l_ptr.p, nsz * sizeof(int), hipMemcpyDeviceToDevice, s));
+    launch_transpose_scatter<T>(ns, h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.as<T>(), h->tmp_b.as<int>(),
+                                h->l_ind.as<int>(), h->l_val.as<T>(), s);
+    launch_segmented_sort<T>(ns, h->l_ptr.as<int>(), h->l_ind.as<int>(), h->l_val.as<T>(), s);
+
+    // level schedule
+    unsigned long long *d_max = h->ctr.as<unsigned long long>() + kCtrCount - 1;
+    RC_HIP(hipMemsetAsync(d_max, 0, sizeof(unsigned long long), s));
+    launch_level_max(ns, h->level.as<int>(), d_max, s);
+    unsigned long long maxlev = 0;
+    RC_HIP(hipMemcpyAsync(&maxlev, d_max, sizeof(maxlev), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    const int nl = ns > 0 ? (int)maxlev + 1 : 0;
+    h->nlevels = nl;
+    const size_t nlsz = (size_t)nl + 1;
+    RC_HIP(h->tmp_a.ensure(std::max(nsz, nlsz * 2) * sizeof(int)));
+    RC_HIP(h->tmp_b.ensure(std::max(nsz, nlsz * 3) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure((scan_tmp_elems(std::max(ns, nl) + 1)) * sizeof(int) + 64));
+    int *lcnt = h->tmp_a.as<int>();
+    int *lptr = h->tmp_b.as<int>();
+    int *lpos = lptr + nlsz;
+    int *cptr = lpos + nlsz;
+    RC_HIP(hipMemsetAsync(lcnt, 0, nlsz * sizeof(int), s));
+    launch_level_count(ns, h->level.as<int>(), lcnt, s);
+    scan_exclusive(lcnt, lptr, nl, h->tmp_scan.as<int>(), s);
+    RC_HIP(hipMemcpyAsync(lpos, lptr, nlsz * sizeof(int), hipMemcpyDeviceToDevice, s));
+    RC_HIP(h->order.ensure((size_t)std::max(ns, 1) * sizeof(int)));
+    launch_level_scatter(ns, h->level.as<int>(), lpos, h->order.as<int>(), s);
+    launch_chunk_count(nl, lptr, lcnt, s);
+    scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
+    int nchunks = 0;
+    RC_HIP(hipMemcpyAsync(&nchunks, cptr + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    h->nchunks = nchunks;
+    RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
+    RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
+    launch_chunk_fill(nchunks, nl, lptr, cptr, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipEventRecord(e1, s));
+    RC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = ms;
+    return RC_OK;
+}
+
+extern "C" {
+
+static double factor_bytes(rc_handle_t h)
+{
+    // 12 nnz_low + 4 (n+1) + 2 (4+s) F + (4+s) nnz_off + s n, s = 8 (elimination in FP64)
+    const double s = 8.0;
+    double F = (double)h->fstats.fills;
+    double off = (double)(h->fstats.nnz_factor - h->nf);
+    return 12.0 * (double)h->nnz_low + 4.0 * (double)(h->nf + 1) + 2.0 * (4.0 + s) * F + (4.0 + s) * off +
+           s * (double)h->nf;
+}
+
+int rc_factor(rc_handle_t h, rc_factor_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = prepare(h);
+    if (rc != RC_OK) return rc;
+    std::memset(&h->fstats, 0, sizeof(h->fstats));
+    double fms = 0.0, ams = 0.0;
+    int regrow = 0;
+    h->factored = false;
+    invalidate_graph(h);
+    rc = run_elimination(h, &fms, &regrow);
+    if (rc != RC_OK) return rc;
+    h->prec = h->opt.factor_precision;
+    rc = (h->prec == 32) ? assemble<float>(h, &ams) : assemble<double>(h, &ams);
+    if (rc != RC_OK) return rc;
+    h->fstats.n = h->nf;
+    h->fstats.nnz_lower = h->nnz_low;
+    h->fstats.nlevels = h->nlevels;
+    h->fstats.factor_ms = fms;
+    h->fstats.assemble_ms = ams;
+    h->fstats.pool_regrowths = regrow;
+    h->fstats.bytes_factor = factor_bytes(h);
+    h->factored = true;
+    if (st) *st = h->fstats;
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------ factor export
+static int fetch_raw_factor(rc_handle_t h, std::vector<long long> &start, std::vector<int> &cnt, std::vector<int> &row,
+                            std::vector<double> &val, std::vector<double> &diag)
+{
+    const size_t nf = (size_t)h->nf;
+    start.resize(nf);
+    cnt.resize(nf);
+    diag.resize(nf);
+    RC_HIP(hipMemcpyAsync(start.data(), h->out_start.p, nf * sizeof(long long), hipMemcpyDeviceToHost, h->stream));
+    RC_HIP(hipMemcpyAsync(cnt.data(), h->out_cnt.p, nf * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    RC_HIP(hipMemcpyAsync(diag.data(), h->diag64.p, nf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    unsigned long long top = 0;
+    RC_HIP(hipMemcpyAsync(&top, h->ctr.as<unsigned long long>() + kCtrOutTop, sizeof(top), hipMemcpyDeviceToHost, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    row.resize((size_t)top);
+    val.resize((size_t)top);
+    if (top) {
+        RC_HIP(hipMemcpyAsync(row.data(), h->out_row.p, top * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipMemcpyAsync(val.data(), h->out_val.p, top * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipStreamSynchronize(h->stream));
+    }
+    return RC_OK;
+}
+
+int rc_get_factor(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *rowptr, int *colind, double *val, double *diag)
+{
+    RC_CHECK_HANDLE();
+    if (!h->factored) return fail(h, RC_ERR_STATE, "no factor");
+    const int ns = h->ns;
+    if (n) *n = ns;
+    if (nnz) *nnz = h->nnz_lt + ns;
+    if (!rowptr && !colind && !val && !diag) return RC_OK;
+    std::vector<int> ptr((size_t)ns + 1), ind((size_t)h->nnz_lt);
+    std::vector<double> v((size_t)h->nnz_lt), d((size_t)ns);
+    RC_HIP(hipMemcpyAsync(ptr.data(), h->lt_ptr.p, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (h->nnz_lt)
+        RC_HIP(hipMemcpyAsync(ind.data(), h->lt_ind.p, ind.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (h->prec == 64) {
+        if (h->nnz_lt) RC_HIP(hipMemcpyAsync(v.data(), h->lt_val.p, v.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipMemcpyAsync(d.data(), h->diagT.p, d.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipStreamSynchronize(h->stream));
+    } else {
+        std::vector<float> vf(v.size()), df(d.size());
+        if (h->nnz_lt) RC_HIP(hipMemcpyAsync(vf.data(), h->lt_val.p, vf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipMemcpyAsync(df.data(), h->diagT.p, df.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        RC_HIP(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < v.size(); ++i) v[i] = vf[i];
+        for (size_t i = 0; i < d.size(); ++i) d[i] = df[i];
+    }
+    int64_t q = 0;
+    if (rowptr) rowptr[0] = 0;
+    for (int k = 0; k < ns; ++k) {
+        if (colind) colind[q] = k;                     // unit diagonal first
+        if (val) val[q] = 1.0;
+        ++q;
+        for (int e = ptr[(size_t)k]; e < ptr[(size_t)k + 1]; ++e, ++q) {
+            if (colind) colind[q] = ind[(size_t)e];
+            if (val) val[q] = v[(size_t)e];
+        }
+        if (rowptr) rowptr[k + 1] = q;
+    }
+    if (diag) std::memcpy(diag, d.data(), d.size() * sizeof(double));
+    return RC_OK;
+}
+
+int rc_dump_factor(rc_handle_t h, const char *prefix)
+{
+    RC_CHECK_HANDLE();
+    if (!prefix) return fail(h, RC_ERR_ARG, "null prefix");
+    if (!h->factored) return fail(h, RC_ERR_STATE, "no factor");
+    std::vector<long long> start;
+    std::vector<int> cnt, row;
+    std::vector<double> val, diag;
+    int rc = fetch_raw_factor(h, start, cnt, row, val, diag);
+    if (rc != RC_OK) return rc;
+    // CSC of L with the unit diagonal first (driver_local.cpp:881-934, 969-983)
+    const int nf = h->nf;
+    std::vector<int64_t> cp((size_t)nf + 1, 0);
+    for (int k = 0; k < nf; ++k) cp[(size_t)k + 1] = cp[(size_t)k] + cnt[(size_t)k] + 1;
+    std::vector<int> ri((size_t)cp[(size_t)nf]);
+    std::vector<double> vv((size_t)cp[(size_t)nf]);
+    for (int k = 0; k < nf; ++k) {
+        int64_t q = cp[(size_t)k];
+        ri[(size_t)q] = k;
+        vv[(size_t)q] = 1.0;
+        ++q;
+        for (int t = 0; t < cnt[(size_t)k]; ++t, ++q) {
+            ri[(size_t)q] = row[(size_t)(start[(size_t)k] + t)];
+            vv[(size_t)q] = val[(size_t)(start[(size_t)k] + t)];
+        }
+    }
+    std::string p(prefix);
+    rc = write_mtx_csc(p + "l_sol.mtx", nf, nf, cp.data(), ri.data(), vv.data(), h->err);
+    if (rc != RC_OK) return rc;
+    FILE *f = fopen((p + "sol_diag.txt").c_str(), "wb");
+    if (!f) return fail(h, RC_ERR_IO, "cannot write " + p + "sol_diag.txt");
+    for (int k = 0; k < nf; ++k) fprintf(f, "%.17g\n", diag[(size_t)k]);
+    fclose(f);
+    return RC_OK;
+}
+
+int rc_load_factor(rc_handle_t h, const char *prefix)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!prefix) return fail(h, RC_ERR_ARG, "null prefix");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = prepare(h);
+    if (rc != RC_OK) return rc;
+    std::string p(prefix);
+    MtxTriplets t;
+    rc = read_mtx(p + "l_sol.mtx", t, h->err);
+    if (rc != RC_OK) return rc;
+    const int nf = h->nf;
+    if (t.nrows != nf || t.ncols != nf) return fail(h, RC_ERR_FORMAT, "factor dimension does not match the system");
+    std::vector<double> diag;
+    {
+        std::ifstream in(p + "sol_diag.txt");
+        if (!in) return fail(h, RC_ERR_IO, "cannot open " + p + "sol_diag.txt");
+        double v;
+        while (in >> v) diag.push_back(v);
+    }
+    if ((int)diag.size() != nf) return fail(h, RC_ERR_FORMAT, "sol_diag.txt length does not match");
+    // columns of L, strict lower entries, sorted by row
+    std::vector<int> cnt((size_t)nf, 0);
+    for (size_t e = 0; e < t.rows.size(); ++e) {
+        if (t.rows[e] < t.cols[e]) return fail(h, RC_ERR_FORMAT, "factor entry above the diagonal");
+        if (t.rows[e] > t.cols[e]) cnt[(size_t)t.cols[e]]++;
+    }
+    std::vector<long long> start((size_t)nf, 0);
+    long long tot = 0;
+    for (int k = 0; k < nf; ++k) { start[(size_t)k] = tot; tot += cnt[(size_t)k]; }
+    std::vector<int> row((size_t)tot);
+    std::vector<double> val((size_t)tot);
+    std::vector<long long> pos(start);
+    for (size_t e = 0; e < t.rows.size(); ++e)
+        if (t.rows[e] > t.cols[e]) {
+            long long q = pos[(size_t)t.cols[e]]++;
+            row[(size_t)q] = t.rows[e];
+            val[(size_t)q] = t.vals[e];
+        }
+    std::vector<int> level((size_t)nf, 0);
+    for (int k = 0; k < nf; ++k) {
+        std::vector<std::pair<int, double>> col((size_t)cnt[(size_t)k]);
+        for (int i = 0; i < cnt[(size_t)k]; ++i) col[(size_t)i] = {row[(size_t)(start[(size_t)k] + i)], val[(size_t)(start[(size_t)k] + i)]};
+        std::sort(col.begin(), col.end());
+        for (int i = 0; i < cnt[(size_t)k]; ++i) {
+            row[(size_t)(start[(size_t)k] + i)] = col[(size_t)i].first;
+            val[(size_t)(start[(size_t)k] + i)] = col[(size_t)i].second;
+            int r = col[(size_t)i].first;
+            level[(size_t)r] = std::max(level[(size_t)r], level[(size_t)k] + 1);
+        }
+    }
+    if (h->pool_cap == 0) {
+        h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low));
+        h->scratch_cap = 2048;
+    }
+    h->out_cap = std::max<long long>(h->out_cap, tot + 16);
+    rc = alloc_factor_work(h);
+    if (rc != RC_OK) return rc;
+    RC_HIP(hipMemcpyAsync(h->out_start.p, start.data(), (size_t)nf * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipMemcpyAsync(h->out_cnt.p, cnt.data(), (size_t)nf * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipMemcpyAsync(h->diag64.p, diag.data(), (size_t)nf * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipMemcpyAsync(h->level.p, level.data(), (size_t)nf * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (tot) {
+        RC_HIP(hipMemcpyAsync(h->out_row.p, row.data(), (size_t)tot * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        RC_HIP(hipMemcpyAsync(h->out_val.p, val.data(), (size_t)tot * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    unsigned long long ctr[kCtrCount] = {0};
+    ctr[kCtrOutTop] = (unsigned long long)tot;
+    RC_HIP(hipMemcpyAsync(h->ctr.p, ctr, sizeof(ctr), hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    std::memset(&h->fstats, 0, sizeof(h->fstats));
+    h->fstats.nnz_factor = tot + nf;
+    int64_t fills = 0, maxd = 0;
+    int zp = 0;
+    for (int k = 0; k < nf; ++k) {
+        fills += std::max(cnt[(size_t)k] - 1, 0);
+        maxd = std::max<int64_t>(maxd, cnt[(size_t)k]);
+        if (diag[(size_t)k] == 0.0) ++zp;
+    }
+    h->fstats.fills = fills;
+    h->fstats.max_degree = maxd;
+    h->fstats.zero_pivots = zp;
+    h->prec = h->opt.factor_precision;
+    double ams = 0.0;
+    invalidate_graph(h);
+    rc = (h->prec == 32) ? assemble<float>(h, &ams) : assemble<double>(h, &ams);
+    if (rc != RC_OK) return rc;
+    h->fstats.n = nf;
+    h->fstats.nnz_lower = h->nnz_low;
+    h->fstats.nlevels = h->nlevels;
+    h->fstats.assemble_ms = ams;
+    h->factored = true;
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------ solve
+static SolveArgs solve_args(rc_handle_t h)
+{
+    SolveArgs a;
+    a.n = h->ns;
+    a.a_ptr = h->a_ptr.as<int>();
+    a.a_ind = h->a_ind.as<int>();
+    a.a_val = h->a_val.as<double>();
+    a.nchunks = h->nchunks;
+    a.chunk_start = h->chunk_start.as<int>();
+    a.chunk_cnt = h->chunk_cnt.as<int>();
+    a.order = h->order.as<int>();
+    a.l_ptr = h->l_ptr.as<int>();
+    a.l_ind = h->l_ind.as<int>();
+    a.lt_ptr = h->lt_ptr.as<int>();
+    a.lt_ind = h->lt_ind.as<int>();
+    a.l_val = h->l_val.p;
+    a.lt_val = h->lt_val.p;
+    a.diag = h->diagT.p;
+    a.x = h->x.as<double>();
+    a.r = h->r.as<double>();
+    a.ap = h->ap.as<double>();
+    a.p0 = h->p0.as<double>();
+    a.p1 = h->p1.as<double>();
+    a.y = h->y.p;
+    a.z = h->z.p;
+    a.sc = h->scal.as<PcgScalars>();
+    a.partials = h->partials.as<double>();
+    a.ctr_fwd = h->ctrs.as<unsigned int>();
+    a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
+    a.reduce_blocks = h->reduce_blocks;
+    a.spin_ticks = h->spin_ticks;
+    a.b = h->b.as<double>();
+    return a;
+}
+
+static int alloc_solve(rc_handle_t h)
+{
+    const size_t n = (size_t)std::max(h->ns, 1);
+    const size_t s = h->prec == 32 ? 4 : 8;
+    const size_t old_y = h->y.p ? (size_t)h->y.p : 0;
+    RC_HIP(h->x.ensure(n * 8));
+    RC_HIP(h->r.ensure(n * 8));
+    RC_HIP(h->ap.ensure(n * 8));
+    RC_HIP(h->p0.ensure(n * 8));
+    RC_HIP(h->p1.ensure(n * 8));
+    RC_HIP(h->b.ensure(n * 8));
+    RC_HIP(h->bu.ensure(n * 8));
+    RC_HIP(h->y.ensure(n * s));
+    RC_HIP(h->z.ensure(n * s));
+    RC_HIP(h->scal.ensure(sizeof(PcgScalars)));
+    RC_HIP(h->partials.ensure((size_t)h->reduce_blocks * 2 * sizeof(double)));
+    RC_HIP(h->ctrs.ensure(4 * sizeof(unsigned int)));
+    if ((size_t)h->y.p != old_y) invalidate_graph(h);
+    return RC_OK;
+}
+
+int rc_make_rhs(rc_handle_t h, int kind, uint64_t seed, double *b)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0 || !b) return fail(h, RC_ERR_ARG, "no matrix or null array");
+    const int n = h->n_user;
+    if (kind == RC_RHS_ZERO_SUM_UNIFORM) {                  // solver.hpp:43-61
+        std::mt19937 gen((unsigned long)seed);
+        std::uniform_real_distribution<double> dist(0.0, 1.0);
+        double sum = 0.0;
+        for (int i = 0; i < n; ++i) { b[i] = dist(gen); sum += b[i]; }
+        double mean = sum / n;
+        for (int i = 0; i < n; ++i) b[i] -= mean;
+        return RC_OK;
+    }
+    if (kind != RC_RHS_NORMAL && kind != RC_RHS_NORMAL_PROJECTED) return fail(h, RC_ERR_ARG, "unknown rhs kind");
+    for (int i = 0; i < n; ++i) {
+        U4 o = philox_keyed(seed, (uint32_t)i, 0u, kTagNormal);
+        double u1 = u01_open0(o.x, o.y), u2 = u01_open1(o.z, o.w);
+        b[i] = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+    if (kind == RC_RHS_NORMAL_PROJECTED) {
+        Csc user;
+        if (h->kind == RC_INPUT_GROUNDED) trim_last(h->base, user);
+        const Csc &g = h->kind == RC_INPUT_GROUNDED ? user : h->base;
+        std::vector<int> comp;
+        int nc = components(g, comp);
+        std::vector<double> s((size_t)nc, 0.0), c((size_t)nc, 0.0);
+        for (int i = 0; i < n; ++i) { s[(size_t)comp[(size_t)i]] += b[i]; c[(size_t)comp[(size_t)i]] += 1.0; }
+        for (int i = 0; i < n; ++i) b[i] -= s[(size_t)comp[(size_t)i]] / c[(size_t)comp[(size_t)i]];
+    }
+    return RC_OK;
+}
+
+int rc_set_reference_solution(rc_handle_t h, const double *x_ref)
+{
+    RC_CHECK_HANDLE();
+    if (!x_ref) { h->xref.clear(); return RC_OK; }
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    h->xref.assign(x_ref, x_ref + h->n_user);
+    return RC_OK;
+}
+
+static int build_graph(rc_handle_t h)
+{
+    if (h->graph && h->graph_prec == h->prec) return RC_OK;
+    invalidate_graph(h);
+    SolveArgs a = solve_args(h);
+    hipGraph_t g;
+    RC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    if (h->prec == 32) {
+        launch_pcg_iteration<float>(a, 0, h->stream);
+        launch_pcg_iteration<float>(a, 1, h->stream);
+    } else {
+        launch_pcg_iteration<double>(a, 0, h->stream);
+        launch_pcg_iteration<double>(a, 1, h->stream);
+    }
+    RC_HIP(hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->graph = nullptr;
+        return fail(h, RC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    h->graph_prec = h->prec;
+    return RC_OK;
+}
+
+static int set_rhs_device(rc_handle_t h, const double *b_user)
+{
+    int rc = alloc_solve(h);
+    if (rc != RC_OK) return rc;
+    RC_HIP(hipMemcpyAsync(h->bu.p, b_user, (size_t)h->ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    launch_gather(h->bu.as<double>(), h->perm_d.as<int>(), h->b.as<double>(), h->ns, h->stream);
+    RC_HIP(hipGetLastError());
+    h->rhs_set = true;
+    return RC_OK;
+}
+
+static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    int rc = alloc_solve(h);
+    if (rc != RC_OK) return rc;
+    hipStream_t s = h->stream;
+    PcgScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.tol = tol;
+    sc.maxit = h->opt.maxit;
+    sc.stop_rule = h->opt.stop_rule;
+    hipEvent_t e0, e1, e2;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    RC_HIP(hipEventCreate(&e2));
+    rc = build_graph(h);
+    if (rc != RC_OK) return rc;
+    SolveArgs a = solve_args(h);
+    RC_HIP(hipEventRecord(e0, s));
+    RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
+    if (h->prec == 32) launch_pcg_init<float>(a, s);
+    else launch_pcg_init<double>(a, s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipEventRecord(e1, s));
+    // iterations: 2 per graph launch; poll the done flag every 4 launches
+    int *done_h = nullptr;
+    RC_HIP(hipHostMalloc((void **)&done_h, sizeof(int) * 2, hipHostMallocDefault));
+    const int per_poll = 4;
+    int launched = 0;
+    const int max_launch = (h->opt.maxit + 1) / 2 + per_poll;
+    done_h[0] = 0;
+    while (launched < max_launch) {
+        for (int t = 0; t < per_poll && launched < max_launch; ++t, ++launched) {
+            hipError_t e = hipGraphLaunch(h->graph, s);
+            if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e)); }
+        }
+        hipError_t e = hipMemcpyAsync(done_h, (char *)h->scal.p + offsetof(PcgScalars, done), sizeof(int),
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
+        if (done_h[0]) break;
+    }
+    (void)hipHostFree(done_h);
+    RC_HIP(hipEventRecord(e2, s));
+    launch_residual(a, s);
+    RC_HIP(hipMemcpyAsync(&sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    float ms_setup = 0.f, ms_pcg = 0.f;
+    (void)hipEventElapsedTime(&ms_setup, e0, e1);
+    (void)hipEventElapsedTime(&ms_pcg, e1, e2);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (st) {
+        st->iterations = sc.iter;
+        st->converged = (sc.status == 0 && sc.done && sc.iter < h->opt.maxit) ? 1 : 0;
+        if (sc.status == 0 && sc.done && sc.iter == h->opt.maxit) {
+            bool met = h->opt.stop_rule == RC_STOP_REFERENCE ? (std::sqrt(sc.rz) / std::sqrt(sc.rz0) <= tol)
+                                                              : (std::sqrt(sc.rr) / std::sqrt(sc.bb) <= tol);
+            st->converged = met ? 1 : 0;
+        }
+        st->rel_res_true = sc.bnorm2 > 0.0 ? std::sqrt(sc.res2) / std::sqrt(sc.bnorm2) : 0.0;
+        st->rel_res_precond = sc.rz0 > 0.0 ? std::sqrt(std::max(sc.rz, 0.0)) / std::sqrt(sc.rz0) : 0.0;
+        st->pcg_ms = ms_pcg;
+        st->setup_ms = ms_setup;
+        st->err_vs_oracle = -1.0;
+    }
+    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
+    if (sc.status & kPcgBreakdown) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown (p'Ap <= 0)");
+    return RC_OK;
+}
+
+static int fetch_solution(rc_handle_t h, double *x_user)
+{
+    // x_user[perm[i]] = x[i]
+    launch_scatter(h->x.as<double>(), h->perm_d.as<int>(), h->bu.as<double>(), h->ns, h->stream);
+    RC_HIP(hipMemcpyAsync(x_user, h->bu.p, (size_t)h->ns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    return RC_OK;
+}
+
+static double a_norm_err(rc_handle_t h, const double *x)
+{
+    // ||x - xref||_A / ||xref||_A on the user system (host, diagnostics only)
+    Csc user;
+    if (h->kind == RC_INPUT_GROUNDED) trim_last(h->base, user);
+    const Csc &g = h->kind == RC_INPUT_GROUNDED ? user : h->base;
+    const int n = h->n_user;
+    std::vector<double> d((size_t)n);
+    for (int i = 0; i < n; ++i) d[(size_t)i] = x[i] - h->xref[(size_t)i];
+    double num = 0.0, den = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double ad = 0.0, ax = 0.0;
+        for (int64_t p = g.ptr[(size_t)j]; p < g.ptr[(size_t)j + 1]; ++p) {
+            ad += g.val[(size_t)p] * d[(size_t)g.ind[(size_t)p]];
+            ax += g.val[(size_t)p] * h->xref[(size_t)g.ind[(size_t)p]];
+        }
+        num += d[(size_t)j] * ad;
+        den += h->xref[(size_t)j] * ax;
+    }
+    return den > 0.0 ? std::sqrt(std::max(num, 0.0) / den) : -1.0;
+}
+
+int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!b || !x || !(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad solve arguments");
+    if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_solve");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = set_rhs_device(h, b);
+    if (rc != RC_OK) return rc;
+    rc_solve_stats local;
+    rc_solve_stats *sp = st ? st : &local;
+    rc = solve_resident(h, tol, sp);
+    if (rc != RC_OK && rc != RC_ERR_BREAKDOWN) return rc;
+    int rc2 = fetch_solution(h, x);
+    if (rc2 != RC_OK) return rc2;
+    if (!h->xref.empty()) sp->err_vs_oracle = a_norm_err(h, x);
+    return rc;
+}
+
+// ------------------------------------------------------------------ timing
+int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and solve once before timing");
+    if (reps <= 0 || !ms_per_launch) return fail(h, RC_ERR_ARG, "bad timing arguments");
+    RC_HIP(hipSetDevice(h->opt.device));
+    hipStream_t s = h->stream;
+    SolveArgs a = solve_args(h);
+    PcgScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.tol = 0.0;
+    sc.maxit = 1 << 30;
+    sc.alpha = 0.0;
+    sc.rz = 1.0;
+    sc.rz0 = 1.0;
+    RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
+    const double n = (double)h->ns, nnzA = (double)h->sys.nnz(), nnzL = (double)h->nnz_lt;
+    const double sz = h->prec == 32 ? 4.0 : 8.0;
+    const double b_spmv = 12.0 * nnzA + 4.0 * (n + 1) + 16.0 * n;
+    const double b_tri = (4.0 + sz) * nnzL + 4.0 * (n + 1) + 2.0 * sz * n;
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    double total = 0.0;
+    auto fill_y = [&]() {
+        if (h->prec == 32) launch_fill_sentinel<float>((float *)h->y.p, h->ns, s);
+        else launch_fill_sentinel<double>((double *)h->y.p, h->ns, s);
+    };
+    auto fill_z = [&]() {
+        if (h->prec == 32) launch_fill_sentinel<float>((float *)h->z.p, h->ns, s);
+        else launch_fill_sentinel<double>((double *)h->z.p, h->ns, s);
+    };
+    for (int t = 0; t < reps; ++t) {
+        float ms = 0.f;
+        if (kernel == RC_KERNEL_SPMV) {
+            RC_HIP(hipEventRecord(e0, s));
+            if (h->prec == 32) launch_spmv_only<float>(a, s); else launch_spmv_only<double>(a, s);
+            RC_HIP(hipEventRecord(e1, s));
+        } else if (kernel == RC_KERNEL_TRSV_FWD) {
+            fill_y();
+            launch_reset_counters(a.ctr_fwd, a.ctr_bwd, s);
+            RC_HIP(hipEventRecord(e0, s));
+            if (h->prec == 32) launch_trsv_fwd_only<float>(a, s); else launch_trsv_fwd_only<double>(a, s);
+            RC_HIP(hipEventRecord(e1, s));
+        } else if (kernel == RC_KERNEL_TRSV_BWD) {
+            fill_y();
+            fill_z();
+            launch_reset_counters(a.ctr_fwd, a.ctr_bwd, s);
+            if (h->prec == 32) launch_trsv_fwd_only<float>(a, s); else launch_trsv_fwd_only<double>(a, s);
+            RC_HIP(hipEventRecord(e0, s));
+            if (h->prec == 32) launch_trsv_bwd_only<float>(a, s); else launch_trsv_bwd_only<double>(a, s);
+            RC_HIP(hipEventRecord(e1, s));
+        } else if (kernel == RC_KERNEL_PCG_ITER) {
+            RC_HIP(hipEventRecord(e0, s));
+            if (h->prec == 32) launch_pcg_iteration<float>(a, t & 1, s); else launch_pcg_iteration<double>(a, t & 1, s);
+            RC_HIP(hipEventRecord(e1, s));
+        } else {
+            return fail(h, RC_ERR_ARG, "unknown kernel id");
+        }
+        RC_HIP(hipGetLastError());
+        RC_HIP(hipEventSynchronize(e1));
+        RC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned int stat = 0;
+    RC_HIP(hipMemcpy(&stat, (char *)h->scal.p + offsetof(PcgScalars, status), sizeof(stat), hipMemcpyDeviceToHost));
+    if (stat & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "timed kernel wait expired");
+    *ms_per_launch = total / reps;
+    if (bytes_per_launch) {
+        if (kernel == RC_KERNEL_SPMV) *bytes_per_launch = b_spmv;
+        else if (kernel == RC_KERNEL_PCG_ITER) *bytes_per_launch = b_spmv + 2.0 * b_tri + sz * n + 72.0 * n;
+        else *bytes_per_launch = b_tri;
+    }
+    return RC_OK;
+}
+
+// Resident-input variants used by the benchmark: the RHS is uploaded and
+// permuted before the timed region, the solution stays on the device.
+int rc_set_rhs(rc_handle_t h, const double *b)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!b) return fail(h, RC_ERR_ARG, "null rhs");
+    if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_set_rhs");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = set_rhs_device(h, b);
+    if (rc != RC_OK) return rc;
+    RC_HIP(hipStreamSynchronize(h->stream));
+    return RC_OK;
+}
+
+int rc_solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and rc_set_rhs must precede");
+    RC_HIP(hipSetDevice(h->opt.device));
+    return solve_resident(h, tol, st);
+}
+
+int rc_get_solution(rc_handle_t h, double *x)
+{
+    RC_CHECK_HANDLE();
+    if (!x) return fail(h, RC_ERR_ARG, "null x");
+    if (!h->rhs_set) return fail(h, RC_ERR_STATE, "no solve");
+    return fetch_solution(h, x);
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""HIP path vs the CPU oracle, through the C ABI, on an MI355X.
+
+The factor is a pure function of (matrix, ordering, seed), so the GPU factor is
+compared bit for bit with the oracle's (FP64 values exactly; FP32 values equal
+to the oracle's values rounded to float).  PCG results are compared within
+floating-point tolerance: the dot products reduce in a different order, so
+iteration counts may differ by one and the iterates agree to ~1e-8 relative
+(FP64 factor) / the FP32 factor's precision.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden_path, read_mtx_py
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _case(wl, name):
+    if name == "grid2d_16_log":
+        return wl.grid_laplacian((16, 16), "loguniform", seed=7), "laplacian"
+    if name == "grid3d_8":
+        return wl.grid_laplacian((8, 8, 8)), "laplacian"
+    if name == "grid3d_20_log":
+        return wl.grid_laplacian((20, 20, 20), "loguniform", seed=7), "laplacian"
+    if name == "rgg_3000":
+        return wl.random_geometric_graph(3000, seed=3), "laplacian"
+    if name == "sddm_7":
+        return wl.grid_sddm((7, 7, 7)), "sddm"
+    if name == "two_comp":
+        return read_mtx_py(golden_path("two_components.mtx")), "laplacian"
+    if name == "star_300":
+        d = 300
+        u = np.zeros(d, np.int64); v = np.arange(1, d + 1)
+        w = np.linspace(0.5, 3.0, d)
+        return wl.laplacian_triplets(d + 1, u, v, w), "laplacian_asis"
+    raise KeyError(name)
+
+
+CASES = ["grid2d_16_log", "grid3d_8", "grid3d_20_log", "rgg_3000", "sddm_7", "two_comp", "star_300"]
+
+
+def _solver(rc, wl, name, prec=64, seed=11, **kw):
+    (n, r, c, v), kind = _case(wl, name)
+    s = rc.Solver(seed=seed, precision=prec, **kw)
+    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if kind == "sddm" else rc.INPUT_LAPLACIAN)
+    s.order(rc.ORDER_AS_IS if kind == "laplacian_asis" else rc.ORDER_NNZ_SORT, 1)
+    return s, (n, r, c, v), kind
+
+
+def _oracle(orc, data, kind, seed=11):
+    n, r, c, v = data
+    order = "as-is" if kind == "laplacian_asis" else "nnz-sort"
+    fac, sys_, perm, appended = orc.prepare_system(n, r, c, v, kind="sddm" if kind == "sddm" else "laplacian",
+                                                   order=order, order_seed=1)
+    f_full = orc.factor(fac, seed=seed)
+    f = orc.factor_trim_last(f_full) if appended else f_full
+    return f_full, f, sys_, perm
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("name", CASES)
+def test_factor_bitwise_equals_oracle(rc, orc, wl, name, prec):
+    say(f"[gpu] factor parity case={name} prec={prec}")
+    s, data, kind = _solver(rc, wl, name, prec)
+    st = s.factor()
+    rp, ci, va, dg = s.get_factor()
+    f_full, f, _, _ = _oracle(orc, data, kind)
+    n = f.n
+    assert len(rp) == n + 1
+    # unit diagonal first, then the oracle's off-diagonal entries
+    assert np.all(ci[rp[:-1]] == np.arange(n)) and np.all(va[rp[:-1]] == 1.0)
+    mask = np.ones(len(ci), bool)
+    mask[rp[:-1]] = False
+    np.testing.assert_array_equal(np.diff(rp) - 1, np.diff(f.ptr))
+    np.testing.assert_array_equal(ci[mask], f.ind)
+    if prec == 64:
+        np.testing.assert_array_equal(va[mask], f.val)
+        np.testing.assert_array_equal(dg, f.diag)
+    else:
+        np.testing.assert_array_equal(va[mask], f.val.astype(np.float32).astype(np.float64))
+        np.testing.assert_array_equal(dg, f.diag.astype(np.float32).astype(np.float64))
+    assert st["nnz_factor"] == f_full.stats["nnz_off"] + f_full.n
+    assert st["fills"] == f_full.stats["fills"]
+    assert st["max_degree"] == f_full.stats["max_degree"]
+    assert st["zero_pivots"] == f_full.stats["zero_pivots"]
+    assert st["nlevels"] == int(f.level.max()) + 1
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("name", ["grid2d_16_log", "grid3d_20_log", "rgg_3000", "sddm_7", "two_comp"])
+def test_pcg_matches_oracle(rc, orc, wl, name, prec):
+    say(f"[gpu] pcg parity case={name} prec={prec}")
+    s, data, kind = _solver(rc, wl, name, prec)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL if kind == "sddm" else rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert so["rc"] == 0
+    assert abs(st["iterations"] - so["iterations"]) <= 1, (st, so)
+    xg = x[perm]
+    tol = 1e-7 if prec == 64 else 1e-4
+    assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
+    assert st["rel_res_true"] <= max(2.0 * so["rel_res_true"], 1e-9)
+
+
+def test_c1_config_iterations_and_residual(rc, orc, wl):
+    """64x64 unit grid, nnz-sort seed 1, FP64, N(0,1) seed 42 projected, tol 1e-6."""
+    say("[gpu] C1 64x64")
+    n, r, c, v = wl.grid_laplacian((64, 64))
+    s = rc.Solver(seed=1, precision=64)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    st_f = s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    fac, sys_, perm, _ = orc.prepare_system(n, r, c, v)
+    f = orc.factor(fac, seed=1)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6)
+    assert st_f["nnz_factor"] == f.stats["nnz_off"] + n
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    assert st["converged"] == 1
+    assert st["rel_res_true"] < 1e-5
+
+
+def test_schedule_independence(rc, wl):
+    """Any workgroup count gives the same factor (work-stealing queue)."""
+    say("[gpu] schedule independence")
+    outs = []
+    for nb in (1, 3, 64, 2048):
+        s, _, _ = _solver(rc, wl, "grid3d_8", 64, num_blocks=nb)
+        s.factor()
+        outs.append(s.get_factor())
+        s.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_pool_regrowth_gives_same_factor(rc, wl):
+    say("[gpu] pool regrowth")
+    s1, _, _ = _solver(rc, wl, "grid3d_20_log", 64)
+    s1.factor()
+    s2, _, _ = _solver(rc, wl, "grid3d_20_log", 64, pool_factor=0.05)
+    st2 = s2.factor()
+    assert st2["pool_regrowths"] >= 1
+    for a, b in zip(s1.get_factor(), s2.get_factor()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_dump_and_reload_factor(rc, wl, tmp_path):
+    say("[gpu] dump/load")
+    s, data, kind = _solver(rc, wl, "sddm_7", 64)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL, 42)
+    x1, st1 = s.solve(b, tol=1e-8)
+    prefix = str(tmp_path / "run_")
+    s.dump_factor(prefix)
+    lines = open(prefix + "l_sol.mtx").read().splitlines()
+    assert lines[0].startswith("%%MatrixMarket matrix coordinate real general")
+    t, _, _ = _solver(rc, wl, "sddm_7", 64)
+    t.load_factor(prefix)
+    x2, st2 = t.solve(b, tol=1e-8)
+    assert st1["iterations"] == st2["iterations"]
+    np.testing.assert_allclose(x1, x2, rtol=0, atol=1e-12 * np.abs(x1).max())
+
+
+def test_resident_solve_and_kernel_timing(rc, wl):
+    say("[gpu] resident solve + timing")
+    s, _, kind = _solver(rc, wl, "grid3d_20_log", 32)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    s.set_rhs(b)
+    st2 = s.solve_resident(1e-6)
+    assert st2["iterations"] == st["iterations"]
+    np.testing.assert_array_equal(s.get_solution(), x)
+    for k in (rc.KERNEL_SPMV, rc.KERNEL_TRSV_FWD, rc.KERNEL_TRSV_BWD, rc.KERNEL_PCG_ITER):
+        ms, by = s.time_kernel(k, reps=3)
+        assert ms > 0 and by > 0
