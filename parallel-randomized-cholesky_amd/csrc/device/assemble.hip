@@ -1,14 +1,19 @@
-// assemble.hip -- factor compaction, transpose and level schedule.
+// assemble.hip -- factor compaction, transpose, level schedule, sliced ELL.
 //
 // Replaces the reference's factor->CSR path (driver.cu:1310-1548, 1743-1870:
 // construct_rowptr, remove_last_csr, shift_element, construct_csr,
 // copy_diagonal, segment_sort_indices, CUB reduce/scan/segmented sort) and the
-// SDDM ground trim (driver_physics.cu:1945-2010) with own kernels: a
-// three-phase device scan, one compaction pass that also drops the ground row
-// and converts to the solve precision, a counting-sort transpose followed by
-// a per-row sort (so every row of L is in ascending column order and the
-// forward-solve sums are deterministic), and a counting sort of rows by level
-// for the triangular solves.
+// SDDM ground trim (driver_physics.cu:1945-2010) with own kernels:
+//   * a three-phase device scan and one compaction pass (CSR of L^T, ground
+//     row dropped, values converted to the solve precision);
+//   * the transpose to rows of L by a stable radix sort of the entries by row
+//     (each row ends up in ascending column order, so forward-solve sums are
+//     deterministic and equal to the oracle's);
+//   * a stable radix sort of rows by (level, descending row length): the
+//     position order in which the triangular solves run and in which every
+//     PCG vector is stored, so a 64-row chunk touches contiguous memory;
+//   * sliced ELL (64 rows per slice, entry-major) for L, L^T and A so every
+//     index/value load of a wave is one coalesced 256-byte access.
 #include "common.cuh"
 #include "kernels.h"
 
@@ -121,96 +126,126 @@ __global__ void k_lt_fill(AssembleArgs a, const int *lt_ptr, int *lt_ind, T *lt_
     }
 }
 
-__global__ void k_count_rows(int nnz, const int *ind, int *cnt)
-{
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += gridDim.x * blockDim.x)
-        atomicAdd(&cnt[ind[e]], 1);
-}
-
-template <typename T>
-__global__ void k_transpose_scatter(int ns, const int *lt_ptr, const int *lt_ind, const T *lt_val, int *pos,
-                                    int *l_ind, T *l_val)
-{
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
-        for (int e = lt_ptr[k]; e < lt_ptr[k + 1]; ++e) {
-            int r = lt_ind[e];
-            int p = atomicAdd(&pos[r], 1);
-            l_ind[p] = k;
-            l_val[p] = lt_val[e];
-        }
-    }
-}
-
-// One wave per segment.  Keys within a segment are distinct column indices.
-// Segments up to kSegLds entries are sorted in LDS, longer ones in place in
-// global memory; both with the ascending bitonic network (no padding).
-constexpr int kSegLds = 1024;
-
-template <typename T>
-__device__ void seg_network(int *key, T *val, int len)
-{
-    const int lane = lane_id();
-    int P = 1;
-    while (P < len) P <<= 1;
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < (P >> 1); t += kWave) {
-                int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                int l = (j == (k >> 1)) ? (i ^ (k - 1)) : (i + j);
-                if (l < len && key[i] > key[l]) {
-                    int ki = key[i]; key[i] = key[l]; key[l] = ki;
-                    T vi = val[i]; val[i] = val[l]; val[l] = vi;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kWave) k_segmented_sort(int nseg, const int *ptr, int *ind, T *val)
-{
-    __shared__ int skey[kSegLds];
-    __shared__ T sval[kSegLds];
-    const int lane = lane_id();
-    for (int sgm = blockIdx.x; sgm < nseg; sgm += gridDim.x) {
-        int b = ptr[sgm], len = ptr[sgm + 1] - b;
-        if (len <= 1) continue;
-        bool sorted = true;
-        for (int t = lane; t + 1 < len; t += kWave)
-            if (ind[b + t] > ind[b + t + 1]) sorted = false;
-        if (!__any(!sorted)) continue;
-        if (len <= kSegLds) {
-            for (int t = lane; t < len; t += kWave) { skey[t] = ind[b + t]; sval[t] = val[b + t]; }
-            __syncthreads();
-            seg_network<T>(skey, sval, len);
-            for (int t = lane; t < len; t += kWave) { ind[b + t] = skey[t]; val[b + t] = sval[t]; }
-            __syncthreads();
-        } else {
-            seg_network<T>(ind + b, val + b, len);
-        }
-    }
-}
-
 __global__ void k_level_max(int ns, const int *level, unsigned long long *out)
 {
     int m = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
         m = level[i] > m ? level[i] : m;
-    atomicMax(out, (unsigned long long)m);
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(m, off, kWave);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0) atomicMax(out, (unsigned long long)m);
 }
 
-__global__ void k_level_count(int ns, const int *level, int *cnt)
+__global__ void k_iota(int *v, int n)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
-        atomicAdd(&cnt[level[i]], 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = i;
 }
 
-__global__ void k_level_scatter(int ns, const int *level, int *pos, int *order)
+__global__ void k_copy_keys(const int *src, uint32_t *dst, int n)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
-        int p = atomicAdd(&pos[level[i]], 1);
-        order[p] = i;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = (uint32_t)src[i];
+}
+
+__global__ void k_row_expand(int ns, const int *ptr, int *rowk)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x)
+        for (int e = ptr[k]; e < ptr[k + 1]; ++e) rowk[e] = k;
+}
+
+__global__ void k_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+        int lo = (i == 0) ? -1 : (int)(keys[i - 1] >> shift);
+        int hi = (i == n) ? nseg : (int)(keys[i] >> shift);
+        if (hi > nseg) hi = nseg;
+        for (int r = lo + 1; r <= hi; ++r) ptr[r] = i;
+    }
+}
+
+__global__ void k_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys)
+{
+    const int lmax = (1 << len_bits) - 1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
+        int len = lptr[k + 1] - lptr[k];
+        if (len > lmax) len = lmax;
+        keys[k] = ((uint32_t)level[k] << len_bits) | (uint32_t)(lmax - len);
+    }
+}
+
+__global__ void k_inverse_perm(int n, const int *order, int *pos_of)
+{
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) pos_of[order[p]] = p;
+}
+
+template <typename T>
+__global__ void k_gather_t(const T *src, const int *idx, T *dst, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[idx[i]];
+}
+
+__global__ void k_mul64(const int *in, int *out, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i] * kWave;
+}
+
+__device__ __forceinline__ void slice_row(int c, const int *start, const int *cnt, const int *order, int nrows,
+                                          int lane, int &row)
+{
+    if (start) {
+        row = (lane < cnt[c]) ? order[start[c] + lane] : -1;
+    } else {
+        int p = c * kWave + lane;
+        row = (p < nrows) ? (order ? order[p] : p) : -1;
+    }
+}
+
+__global__ void k_ell_width(int nslices, const int *start, const int *cnt, const int *order, int nrows,
+                            const int *ptr, int *width)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int c = wave; c < nslices; c += nwaves) {
+        int row;
+        slice_row(c, start, cnt, order, nrows, lane, row);
+        int len = row >= 0 ? ptr[row + 1] - ptr[row] : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            int o = __shfl_xor(len, off, kWave);
+            len = o > len ? o : len;
+        }
+        if (lane == 0) width[c] = len;
+    }
+}
+
+template <typename TS, typename TD>
+__global__ void k_ell_fill(int nslices, const int *start, const int *cnt, const int *order, int nrows, EllSrc src,
+                           const int *off, const int *width, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    const TS *sval = (const TS *)src.val;
+    for (int c = wave; c < nslices; c += nwaves) {
+        int row;
+        slice_row(c, start, cnt, order, nrows, lane, row);
+        int b = row >= 0 ? src.ptr[row] : 0;
+        int len = row >= 0 ? src.ptr[row + 1] - b : 0;
+        const int w = width[c];
+        const int o = off[c];
+        for (int t = 0; t < w; ++t) {
+            int ind = pad_ind;
+            TD v = pad_val;
+            if (t < len) {
+                int e = src.eidx ? src.eidx[b + t] : b + t;
+                int col = src.rowk ? src.rowk[e] : src.ind[e];
+                ind = src.colmap ? src.colmap[col] : col;
+                v = (TD)sval[e];
+            }
+            ell_ind[o + t * kWave + lane] = ind;
+            ell_val[o + t * kWave + lane] = v;
+        }
     }
 }
 
@@ -248,38 +283,64 @@ void launch_lt_fill(const AssembleArgs &a, const int *lt_ptr, int *lt_ind, T *lt
     hipLaunchKernelGGL(k_lt_fill<T>, dim3(2048), dim3(256), 0, s, a, lt_ptr, lt_ind, lt_val, diag_out);
 }
 
-void launch_count_rows(int nnz, const int *ind, int *cnt, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_count_rows, dim3(2048), dim3(256), 0, s, nnz, ind, cnt);
-}
-
-template <typename T>
-void launch_transpose_scatter(int ns, const int *lt_ptr, const int *lt_ind, const T *lt_val, int *pos, int *l_ind,
-                              T *l_val, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_transpose_scatter<T>, dim3(2048), dim3(256), 0, s, ns, lt_ptr, lt_ind, lt_val, pos, l_ind,
-                       l_val);
-}
-
-template <typename T>
-void launch_segmented_sort(int nseg, const int *ptr, int *ind, T *val, hipStream_t s)
-{
-    int g = nseg < 8192 ? (nseg > 0 ? nseg : 1) : 8192;
-    hipLaunchKernelGGL(k_segmented_sort<T>, dim3(g), dim3(kWave), 0, s, nseg, ptr, ind, val);
-}
-
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_level_max, dim3(1024), dim3(256), 0, s, ns, level, out);
 }
-void launch_level_count(int ns, const int *level, int *cnt, hipStream_t s)
+void launch_iota(int *v, int n, hipStream_t s) { hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, v, n); }
+void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_level_count, dim3(1024), dim3(256), 0, s, ns, level, cnt);
+    hipLaunchKernelGGL(k_copy_keys, dim3(2048), dim3(256), 0, s, src, dst, n);
 }
-void launch_level_scatter(int ns, const int *level, int *pos, int *order, hipStream_t s)
+void launch_row_expand(int ns, const int *ptr, int *rowk, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_level_scatter, dim3(1024), dim3(256), 0, s, ns, level, pos, order);
+    hipLaunchKernelGGL(k_row_expand, dim3(2048), dim3(256), 0, s, ns, ptr, rowk);
 }
+void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_segment_bounds, dim3(2048), dim3(256), 0, s, n, keys, shift, nseg, ptr);
+}
+void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_keys, dim3(2048), dim3(256), 0, s, ns, level, lptr, len_bits, keys);
+}
+void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_inverse_perm, dim3(2048), dim3(256), 0, s, n, order, pos_of);
+}
+template <typename T>
+void launch_gather_t(const T *src, const int *idx, T *dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather_t<T>, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
+}
+void launch_gather_i(const int *src, const int *idx, int *dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather_t<int>, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
+}
+void launch_mul64(const int *in, int *out, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mul64, dim3(1024), dim3(256), 0, s, in, out, n);
+}
+void launch_ell_width(int nslices, const int *start, const int *cnt, const int *order, int nrows, const int *ptr,
+                      int *width, hipStream_t s)
+{
+    int g = (nslices + 3) / 4;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_ell_width, dim3(g), dim3(256), 0, s, nslices, start, cnt, order, nrows, ptr, width);
+}
+template <typename TS, typename TD>
+void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *order, int nrows, EllSrc src,
+                     const int *off, const int *width, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val,
+                     hipStream_t s)
+{
+    int g = (nslices + 3) / 4;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL((k_ell_fill<TS, TD>), dim3(g), dim3(256), 0, s, nslices, start, cnt, order, nrows, src, off,
+                       width, pad_ind, pad_val, ell_ind, ell_val);
+}
+
 void launch_chunk_count(int nlevels, const int *lvl_ptr, int *cnt, hipStream_t s)
 {
     hipLaunchKernelGGL(k_chunk_count, dim3(256), dim3(256), 0, s, nlevels, lvl_ptr, cnt);
@@ -293,11 +354,11 @@ void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *
 
 template void launch_lt_fill<float>(const AssembleArgs &, const int *, int *, float *, float *, hipStream_t);
 template void launch_lt_fill<double>(const AssembleArgs &, const int *, int *, double *, double *, hipStream_t);
-template void launch_transpose_scatter<float>(int, const int *, const int *, const float *, int *, int *, float *,
-                                              hipStream_t);
-template void launch_transpose_scatter<double>(int, const int *, const int *, const double *, int *, int *, double *,
-                                               hipStream_t);
-template void launch_segmented_sort<float>(int, const int *, int *, float *, hipStream_t);
-template void launch_segmented_sort<double>(int, const int *, int *, double *, hipStream_t);
+template void launch_gather_t<float>(const float *, const int *, float *, int, hipStream_t);
+template void launch_gather_t<double>(const double *, const int *, double *, int, hipStream_t);
+template void launch_ell_fill<float, float>(int, const int *, const int *, const int *, int, EllSrc, const int *,
+                                           const int *, int, float, int *, float *, hipStream_t);
+template void launch_ell_fill<double, double>(int, const int *, const int *, const int *, int, EllSrc, const int *,
+                                             const int *, int, double, int *, double *, hipStream_t);
 
 }  // namespace rc

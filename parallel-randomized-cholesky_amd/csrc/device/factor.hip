@@ -1,4 +1,4 @@
-// factor.hip -- persistent randomized-elimination kernel for gfx950.
+// factor.hip -- randomized elimination on gfx950, one round per DAG level.
 //
 // Semantics follow cpu_implementation/driver_local.cpp:302-707 and
 // gpu_implementation/driver.cu:43-1302 (the reference's per-column step):
@@ -7,22 +7,26 @@
 // forward cumulative sums C and S = C[d-1], and for every entry i < d-1 draw
 // u in (0,1], pick the first j > i with C[j] - C[i] >= u (S - C[i]) and emit
 // the fill edge (max, min, w_i (S - C[i]) / S) into column min; the factor
-// column holds -w/S and the pivot is S.
+// column holds -w/S and the pivot is S.  The last column is never eliminated
+// (driver.cu:184-197): it keeps count 0 and pivot 0.
 //
 // MI355X design:
-//   * one wave64 (one workgroup) per column at a time; columns come from a
-//     ready queue claimed with one atomic per column (work stealing -- any
-//     grid size is safe, waves never wait while holding work);
+//   * rounds: one launch eliminates every column made ready by the previous
+//     launch (one wave64 per column, grid-stride over the round's list), so
+//     the kernel boundary orders all communication -- no fences, no waits;
 //   * gather, duplicate merge, sorts and the prefix sum run in LDS (global
 //     per-wave scratch for columns above the LDS capacity);
 //   * the sampling draw is Philox4x32-10 keyed by (seed, i, column): with the
 //     canonical (row, weight) merge order and (weight, row) sampling order the
 //     factor is a pure function of (matrix, seed), independent of schedule;
-//   * fill edges are appended to per-column lists in HBM by one atomic
-//     exchange each; dependency counters are agent-scope atomics and the
-//     hand-off of a ready column is a release/acquire pair;
-//   * every wait is bounded; pool or scratch exhaustion raises a status bit,
-//     all waves drain, and the host regrows and reruns (same result).
+//   * each neighbour's dependency counter gets ONE atomic add of
+//     (new fills with it as the larger end) - (retired multiplicity); the
+//     column whose add reaches zero appends the neighbour to the next round's
+//     list with one wave-aggregated atomic;
+//   * fill records and factor entries are carved from per-wave chunks of two
+//     global pools, so pool counters see one atomic per few thousand entries;
+//   * pool or scratch exhaustion raises a status bit, later rounds drain, and
+//     the host regrows and reruns (same result).
 #include "common.cuh"
 #include "kernels.h"
 
@@ -30,12 +34,14 @@ namespace rc {
 
 namespace {
 
-constexpr int kLdsCap = 128;                        // entries per wave in LDS
-constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult
+constexpr int kLdsCap = 128;                                   // entries per wave in LDS
+constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc
+constexpr long long kFillChunk = 2048;
+constexpr long long kOutChunk = 2048;
 
 struct ColBuf {
     double *sw, *mw, *C;
-    int *srow, *smult, *mrow, *mmult;
+    int *srow, *smult, *mrow, *mmult, *minc;
 };
 
 __device__ __forceinline__ ColBuf carve(char *base, int cap)
@@ -48,6 +54,7 @@ __device__ __forceinline__ ColBuf carve(char *base, int cap)
     b.smult = b.srow + cap;
     b.mrow = b.smult + cap;
     b.mmult = b.mrow + cap;
+    b.minc = b.mmult + cap;
     return b;
 }
 
@@ -59,11 +66,11 @@ __device__ __forceinline__ int next_pow2(int v)
 }
 
 // Ascending sorting network (bitonic with a mirrored first comparator in
-// every merge stage), all comparators put the minimum at the lower index, so
+// every merge stage); every comparator puts the minimum at the lower index, so
 // positions >= len behave as +infinity and need no padding.
-// kByRow: key (row, w); otherwise key (w, row).  mult is carried if non-null.
+// kByRow: key (row, w); otherwise key (w, row).  pay is carried if non-null.
 template <bool kByRow>
-__device__ void wave_sort(int *row, double *w, int *mult, int len)
+__device__ void wave_sort(int *row, double *w, int *pay, int len)
 {
     const int lane = lane_id();
     const int P = next_pow2(len);
@@ -80,7 +87,7 @@ __device__ void wave_sort(int *row, double *w, int *mult, int len)
                     if (gt) {
                         row[i] = rl; row[l] = ri;
                         w[i] = wl; w[l] = wi;
-                        if (mult) { int mi = mult[i]; mult[i] = mult[l]; mult[l] = mi; }
+                        if (pay) { int mi = pay[i]; pay[i] = pay[l]; pay[l] = mi; }
                     }
                 }
             }
@@ -91,16 +98,41 @@ __device__ void wave_sort(int *row, double *w, int *mult, int len)
 
 __device__ __forceinline__ void raise_status(unsigned long long *ctr, unsigned long long bits)
 {
-    __hip_atomic_fetch_or(&ctr[kCtrStatus], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicOr(&ctr[kCtrStatus], bits);
 }
 
-// Eliminates column k.  Returns false if the kernel must stop (status set).
-__device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
+struct Cursor {
+    long long cur, end;
+};
+
+// Wave-uniform carve of `count` entries from a per-wave chunk of a pool.
+__device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long chunk, unsigned long long *top,
+                                           long long cap, long long &base)
+{
+    if (c.cur + count > c.end) {
+        long long size = count > chunk ? count : chunk;
+        long long b = 0;
+        if (lane_id() == 0) b = (long long)atomicAdd(top, (unsigned long long)size);
+        b = bcast(b, 0);
+        if (b + size > cap) return false;
+        c.cur = b;
+        c.end = b + size;
+    }
+    base = c.cur;
+    c.cur += count;
+    return true;
+}
+
+// Eliminates column k; newly ready neighbours are appended to `next`.
+// Returns false if the pools or the scratch are exhausted (status set).
+__device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, Cursor &fcur, Cursor &ocur, int *next,
+                                 unsigned int *next_count)
 {
     const int lane = lane_id();
+    const int last = a.n - 1;
     const int lo0 = a.lo_ptr[k];
     const int norig = a.lo_ptr[k + 1] - lo0;
-    const int nf_in = ld_relaxed(&a.nfill[k]);
+    const int nf_in = a.nfill[k];
     const int m = norig + nf_in;
 
     if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
@@ -108,7 +140,6 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
             a.out_start[k] = 0;
             a.out_cnt[k] = 0;
             a.diag[k] = 0.0;
-            fetch_add(&a.ctr[kCtrZeroPivots], 1ull);
         }
         return true;
     }
@@ -125,7 +156,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
         cap = a.scratch_cap;
     } else {
         if (lane == 0) {
-            fetch_max(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
+            atomicMax(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
             raise_status(a.ctr, kStatusScratch);
         }
         return false;
@@ -139,7 +170,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
         b.smult[t] = 1;
     }
     if (lane == 0) {
-        int f = ld_relaxed(&a.head[k]);
+        int f = a.head[k];
         int t = 0;
         while (f >= 0 && t < nf_in) {
             FillRec rec = a.pool[f];
@@ -179,18 +210,21 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
             b.mrow[pos] = r;
             b.mw[pos] = ws;
             b.mmult[pos] = ms;
+            b.minc[pos] = 0;
         }
         d += __popcll(mask);
     }
     __syncthreads();
 
-    // ---- 3. sort merged entries by (w, row); cumulative sums (:573-593)
+    // ---- 3. sort merged entries by (w, row), carrying the merged index;
+    //         forward cumulative sums (:573-593)
     for (int t = lane; t < d; t += kWave) {
         b.srow[t] = b.mrow[t];
         b.sw[t] = b.mw[t];
+        b.smult[t] = t;
     }
     __syncthreads();
-    wave_sort<false>(b.srow, b.sw, nullptr, d);
+    wave_sort<false>(b.srow, b.sw, b.smult, d);
     if (lane == 0) {
         double c = b.sw[0];
         b.C[0] = c;
@@ -205,13 +239,9 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
     // ---- 4. sampled fills (:598-631; gpu auxilliary.hpp:543-592)
     const int nfill_out = d - 1;
     long long fbase = 0;
-    if (nfill_out > 0) {
-        if (lane == 0) fbase = (long long)fetch_add(&a.ctr[kCtrFillTop], (unsigned long long)nfill_out);
-        fbase = bcast(fbase, 0);
-        if (fbase + nfill_out > a.pool_cap) {
-            if (lane == 0) raise_status(a.ctr, kStatusFillPool);
-            return false;
-        }
+    if (nfill_out > 0 && !carve_pool(fcur, nfill_out, kFillChunk, &a.ctr[kCtrFillTop], a.pool_cap, fbase)) {
+        if (lane == 0) raise_status(a.ctr, kStatusFillPool);
+        return false;
     }
     for (int i = lane; i < nfill_out; i += kWave) {
         double u = sample_uniform(a.seed, (uint32_t)i, (uint32_t)k);
@@ -228,53 +258,55 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
         int cc = ri > rj ? rj : ri;
         double fw = (b.sw[i] * rest) / S;
         int rec = (int)(fbase + i);
-        int old = exchange(&a.head[cc], rec);
+        int old = atomicExch(&a.head[cc], rec);
         FillRec fr;
         fr.row = rr;
         fr.next = old;
         fr.w = fw;
         a.pool[rec] = fr;
-        fetch_add(&a.nfill[cc], 1);
-        fetch_add(&a.dep[rr], 1);
+        atomicAdd(&a.nfill[cc], 1);
+        atomicAdd(&b.minc[ri > rj ? b.smult[i] : b.smult[lo]], 1);   // rr gains a dependency
     }
 
     // ---- 5. factor column (row order, -w/S), pivot S, levels (:638-645)
     long long obase = 0;
-    if (lane == 0) obase = (long long)fetch_add(&a.ctr[kCtrOutTop], (unsigned long long)d);
-    obase = bcast(obase, 0);
-    if (obase + d > a.out_cap) {
+    if (!carve_pool(ocur, d, kOutChunk, &a.ctr[kCtrOutTop], a.out_cap, obase)) {
         if (lane == 0) raise_status(a.ctr, kStatusOutPool);
         return false;
     }
-    const int lk = ld_relaxed(&a.level[k]);
+    const int lk = a.level[k];
     for (int i = lane; i < d; i += kWave) {
         int r = b.mrow[i];
         a.out_row[obase + i] = r;
         a.out_val[obase + i] = -(b.mw[i] / S);
-        fetch_max(&a.level[r], lk + 1);
+        if (r != last) atomicMax(&a.level[r], lk + 1);
     }
     if (lane == 0) {
         a.out_start[k] = obase;
         a.out_cnt[k] = d;
         a.diag[k] = S;
-        fetch_add(&a.ctr[kCtrFills], (unsigned long long)nfill_out);
-        fetch_add(&a.ctr[kCtrGathered], (unsigned long long)m);
-        fetch_add(&a.ctr[kCtrNnzOff], (unsigned long long)d);
-        fetch_max(&a.ctr[kCtrMaxDeg], (unsigned long long)d);
-        fetch_max(&a.ctr[kCtrMaxLevel], (unsigned long long)lk);
     }
+    __syncthreads();   // minc complete
 
-    // ---- 6. release, then retire the dependencies (:653-687)
-    fence_release();
-    for (int i = lane; i < d; i += kWave) {
-        int r = b.mrow[i];
-        int mm = b.mmult[i];
-        int old = fetch_add(&a.dep[r], -mm);
-        if (old == mm) {
-            fence_acq_rel();
-            unsigned long long slot = fetch_add(&a.ctr[kCtrQTail], 1ull);
-            if (slot < (unsigned long long)a.n) st_relaxed(&a.queue[slot], r);
-            else raise_status(a.ctr, kStatusInconsistent);   // a column retired twice
+    // ---- 6. retire: one net dependency update per neighbour (:653-687)
+    for (int base_i = 0; base_i < d; base_i += kWave) {
+        int i = base_i + lane;
+        bool ready = false;
+        int r = -1;
+        if (i < d) {
+            r = b.mrow[i];
+            int delta = b.minc[i] - b.mmult[i];
+            if (r != last && delta != 0) {
+                int old = atomicAdd(&a.dep[r], delta);
+                ready = (old + delta == 0);
+            }
+        }
+        uint64_t mask = ballot(ready);
+        if (mask) {
+            unsigned int nb = 0;
+            if (lane == 0) nb = atomicAdd(next_count, (unsigned int)__popcll(mask));
+            nb = bcast(nb, 0);
+            if (ready) next[nb + lanes_below(mask)] = r;
         }
     }
     __syncthreads();
@@ -283,90 +315,141 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds)
 
 }  // namespace
 
-__global__ void __launch_bounds__(kWave) k_factor(FactorArgs a)
+// One elimination round: every column of `cur` (count counts[round]).
+__global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round, const int *cur, int *next)
 {
     __shared__ __attribute__((aligned(16))) char lds[kLdsCap * kBytesPerEntry];
-    const int lane = lane_id();
-    while (true) {
-        long long idx = 0;
-        if (lane == 0) idx = (long long)fetch_add(&a.ctr[kCtrQHead], 1ull);
-        idx = bcast(idx, 0);
-        if (idx >= a.n) break;
-        int col = -1;
-        if (lane == 0) {
-            long long t0 = wall_clock64();
-            while (true) {
-                col = ld_relaxed(&a.queue[idx]);
-                if (col >= 0) break;
-                if (ld_relaxed(&a.ctr[kCtrStatus]) != 0ull) break;
-                if (wall_clock64() - t0 > a.spin_ticks) {
-                    raise_status(a.ctr, kStatusTimeout);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        col = bcast(col, 0);
-        if (col < 0) break;
-        fence_acquire();
-        if (!eliminate_column(a, col, lds)) break;
-        if (ld_relaxed(&a.ctr[kCtrStatus]) != 0ull) break;
+    const unsigned int ncur = a.round_count[round];
+    if (ncur == 0) return;
+    if (a.ctr[kCtrStatus] != 0ull) return;
+    Cursor fcur, ocur;
+    long long *ws = a.wave_state + 4 * (size_t)blockIdx.x;
+    fcur.cur = ws[0]; fcur.end = ws[1];
+    ocur.cur = ws[2]; ocur.end = ws[3];
+    bool ok = true;
+    for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
+        int col = cur[i];
+        if (!eliminate_column(a, col, lds, fcur, ocur, next, &a.round_count[round + 1])) { ok = false; break; }
+    }
+    (void)ok;
+    if (lane_id() == 0) {
+        ws[0] = fcur.cur; ws[1] = fcur.end;
+        ws[2] = ocur.cur; ws[3] = ocur.end;
     }
 }
 
 // ---- set-up kernels ------------------------------------------------------
 
-__global__ void k_factor_init(int n, int *dep, int *head, int *nfill, int *queue, int *level, int *out_cnt)
+__global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, int *nfill, int *level, int *out_cnt,
+                              long long *out_start, double *diag)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        dep[i] = 0;
+        dep[i] = dep_init[i];
         head[i] = -1;
         nfill[i] = 0;
-        queue[i] = -1;
         level[i] = 0;
         out_cnt[i] = 0;
+        out_start[i] = 0;
+        diag[i] = 0.0;
     }
 }
 
-// dep[r] = number of strict-lower entries (r, j) over columns j < r; equal to
-// the reference's count of strict-upper entries of column r
-// (pre_process.hpp:474-496) for a symmetric pattern.
-__global__ void k_factor_deps(int n, const int *lo_ptr, const int *lo_row, int *dep)
+// Round 0: every column without lower-index neighbours, except the last.
+__global__ void k_factor_seed(int n, const int *dep, int *list, unsigned int *count)
 {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        for (int p = lo_ptr[j]; p < lo_ptr[j + 1]; ++p) atomicAdd(&dep[lo_row[p]], 1);
-}
-
-__global__ void k_factor_seed_queue(int n, const int *dep, int *queue, unsigned long long *ctr)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (dep[i] == 0) {
-            int slot = (int)atomicAdd(&ctr[kCtrQTail], 1ull);
-            queue[slot] = i;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        int i = base + threadIdx.x;
+        bool ready = i < n - 1 && dep[i] == 0;
+        uint64_t mask = ballot(ready);
+        if (mask) {
+            unsigned int nb = 0;
+            if (lane_id() == 0) nb = atomicAdd(count, (unsigned int)__popcll(mask));
+            nb = bcast(nb, 0);
+            if (ready) list[nb + lanes_below(mask)] = i;
         }
     }
 }
 
-void launch_factor(const FactorArgs &a, int num_blocks, hipStream_t s)
+void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s)
 {
-    const int g = 1024;
-    hipLaunchKernelGGL(k_factor_init, dim3(g), dim3(256), 0, s, a.n, a.dep, a.head, a.nfill, a.queue, a.level, a.out_cnt);
-    hipLaunchKernelGGL(k_factor_deps, dim3(g), dim3(256), 0, s, a.n, a.lo_ptr, a.lo_row, a.dep);
-    hipLaunchKernelGGL(k_factor_seed_queue, dim3(g), dim3(256), 0, s, a.n, a.dep, a.queue, a.ctr);
-    hipLaunchKernelGGL(k_factor, dim3(num_blocks), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_factor_init, dim3(1024), dim3(256), 0, s, a.n, dep_init, a.dep, a.head, a.nfill, a.level,
+                       a.out_cnt, a.out_start, a.diag);
+    hipLaunchKernelGGL(k_factor_seed, dim3(1024), dim3(256), 0, s, a.n, (const int *)a.dep, a.list0, a.round_count);
+}
+
+void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s)
+{
+    for (int r = first_round; r < first_round + nrounds; ++r) {
+        const int *cur = (r & 1) ? a.list1 : a.list0;
+        int *next = (r & 1) ? a.list0 : a.list1;
+        hipLaunchKernelGGL(k_factor_round, dim3(num_blocks), dim3(kWave), 0, s, a, r, cur, next);
+    }
 }
 
 int factor_lds_capacity() { return kLdsCap; }
 int factor_scratch_bytes_per_entry() { return kBytesPerEntry; }
+long long factor_pool_slack(int num_blocks) { return (long long)num_blocks * (kFillChunk + kOutChunk); }
 
 int factor_occupancy_blocks(int device)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor, kWave, 0) != hipSuccess) per_cu = 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_round, kWave, 0) != hipSuccess) per_cu = 8;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     if (per_cu < 1) per_cu = 1;
     return per_cu * cus;
+}
+
+// Statistics after the factor: nnz_off, fills, max degree, zero pivots
+// (over the first n-1 columns; the last column is a zero pivot by definition).
+__global__ void k_factor_stats(int n, const int *out_cnt, unsigned long long *ctr)
+{
+    unsigned long long nnz = 0, fills = 0, zp = 0, mx = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long d = (unsigned long long)out_cnt[i];
+        nnz += d;
+        fills += d > 0 ? d - 1 : 0;
+        zp += d == 0 ? 1 : 0;
+        mx = d > mx ? d : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nnz += __shfl_xor(nnz, off, kWave);
+        fills += __shfl_xor(fills, off, kWave);
+        zp += __shfl_xor(zp, off, kWave);
+        unsigned long long o = __shfl_xor(mx, off, kWave);
+        mx = o > mx ? o : mx;
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&ctr[kCtrNnzOff], nnz);
+        atomicAdd(&ctr[kCtrFills], fills);
+        atomicAdd(&ctr[kCtrZeroPivots], zp);
+        atomicMax(&ctr[kCtrMaxDeg], mx);
+    }
+}
+
+// Level of the last column (never eliminated): 1 + max level of the columns
+// whose factor column holds it (rows are sorted, so it is their last entry).
+__global__ void k_last_level(int n, const long long *out_start, const int *out_cnt, const int *out_row,
+                             int *level)
+{
+    int m = -1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n - 1; k += gridDim.x * blockDim.x) {
+        int c = out_cnt[k];
+        if (c > 0 && out_row[out_start[k] + c - 1] == n - 1) m = level[k] > m ? level[k] : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(m, off, kWave);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0 && m >= 0) atomicMax(&level[n - 1], m + 1);
+}
+
+void launch_factor_finish(const FactorArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_factor_stats, dim3(1024), dim3(256), 0, s, a.n, (const int *)a.out_cnt, a.ctr);
+    if (a.n > 1)
+        hipLaunchKernelGGL(k_last_level, dim3(1024), dim3(256), 0, s, a.n, (const long long *)a.out_start,
+                           (const int *)a.out_cnt, (const int *)a.out_row, a.level);
 }
 
 }  // namespace rc

@@ -16,9 +16,9 @@ struct FillRec {            // one pending fill edge, 16 B
 
 // Device counter slots (unsigned long long each).
 enum {
-    kCtrQHead = 0, kCtrQTail = 1, kCtrFillTop = 2, kCtrOutTop = 3, kCtrStatus = 4,
-    kCtrFills = 5, kCtrGathered = 6, kCtrMaxDeg = 7, kCtrNnzOff = 8, kCtrMaxLevel = 9,
-    kCtrZeroPivots = 10, kCtrScratchNeed = 11, kCtrCount = 16
+    kCtrFillTop = 2, kCtrOutTop = 3, kCtrStatus = 4,
+    kCtrFills = 5, kCtrNnzOff = 8, kCtrMaxDeg = 7, kCtrZeroPivots = 10, kCtrScratchNeed = 11,
+    kCtrCount = 16
 };
 
 // Device-side status word bits (ctr[kCtrStatus]).
@@ -27,7 +27,6 @@ enum : unsigned long long {
     kStatusFillPool = 1,
     kStatusOutPool = 2,
     kStatusScratch = 4,
-    kStatusTimeout = 8,
     kStatusInconsistent = 16,
 };
 
@@ -36,7 +35,10 @@ struct FactorArgs {
     const int *lo_ptr;      // n+1, strict-lower part of the Laplacian (CSC)
     const int *lo_row;
     const double *lo_w;     // |A[r,k]|
-    int *dep, *head, *nfill, *queue, *level;
+    int *dep, *head, *nfill, *level;
+    int *list0, *list1;     // ready lists of even / odd rounds
+    unsigned int *round_count;   // [round] = columns eliminated in that round
+    long long *wave_state;  // per workgroup: fill cursor, fill end, out cursor, out end
     unsigned long long *ctr;
     FillRec *pool;
     long long pool_cap;
@@ -49,12 +51,14 @@ struct FactorArgs {
     char *scratch;          // per-workgroup overflow buffers
     int scratch_cap;        // entries per workgroup (power of two)
     uint64_t seed;
-    long long spin_ticks;
 };
 
-void launch_factor(const FactorArgs &a, int num_blocks, hipStream_t s);
+void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s);
+void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s);
+void launch_factor_finish(const FactorArgs &a, hipStream_t s);
 int factor_lds_capacity();
 int factor_scratch_bytes_per_entry();
+long long factor_pool_slack(int num_blocks);
 int factor_occupancy_blocks(int device);
 
 // ------------------------------------------------------------ assembly
@@ -75,25 +79,52 @@ struct AssembleArgs {
     const int *level;       // n
 };
 
-// Lt (row k = column k of L, off-diagonal, sorted) in precision T, then its
-// transpose L (row r lists k < r ascending).  Pointers are int32.
+// Stable LSD radix sort of (uint32 key, int value) pairs; result in (k0, v0).
+size_t radix_tmp_ints(int n);
+void radix_sort_pairs(uint32_t *k0, int *v0, uint32_t *k1, int *v1, int n, int key_bits, int *tmp, hipStream_t s);
+
+// Lt (row k = column k of L, off-diagonal, sorted) in precision T.
 void launch_lt_counts(const AssembleArgs &a, int *cnt, hipStream_t s);
 template <typename T>
 void launch_lt_fill(const AssembleArgs &a, const int *lt_ptr, int *lt_ind, T *lt_val, T *diag_out, hipStream_t s);
-void launch_count_rows(int nnz, const int *ind, int *cnt, hipStream_t s);
-template <typename T>
-void launch_transpose_scatter(int ns, const int *lt_ptr, const int *lt_ind, const T *lt_val, int *pos,
-                              int *l_ind, T *l_val, hipStream_t s);
-template <typename T>
-void launch_segmented_sort(int nseg, const int *ptr, int *ind, T *val, hipStream_t s);
 
-// Level schedule: rows grouped by level, levels cut into chunks of <= 64 rows.
+// small helpers
+void launch_iota(int *v, int n, hipStream_t s);
+void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s);
+void launch_row_expand(int ns, const int *ptr, int *rowk, hipStream_t s);
+// ptr[r] = first index i with (keys[i] >> shift) >= r, r in [0, nseg]; keys sorted.
+void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s);
+void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s);
+void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s);
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s);
-void launch_level_count(int ns, const int *level, int *cnt, hipStream_t s);
-void launch_level_scatter(int ns, const int *level, int *pos, int *order, hipStream_t s);
 void launch_chunk_count(int nlevels, const int *lvl_ptr, int *cnt, hipStream_t s);
 void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, int *chunk_start,
                        int *chunk_cnt, hipStream_t s);
+template <typename T> void launch_gather_t(const T *src, const int *idx, T *dst, int n, hipStream_t s);
+void launch_gather_i(const int *src, const int *idx, int *dst, int n, hipStream_t s);
+
+// Sliced ELL (64 rows per slice, entry t of slice row l at off[c] + 64 t + l).
+// Slice c covers rows order[start[c] + l], l < cnt[c] (chunk mode) or
+// positions 64 c + l (tile mode, start == nullptr).  Entries come from CSR
+// (ptr, ind, val) of the row (optionally through eidx: entry q of the row is
+// src entry eidx[ptr[row] + q]) with the column mapped by colmap; padding is
+// (pad_ind, pad_val).
+struct EllSrc {
+    const int *ptr;
+    const int *ind;
+    const void *val;      // T_src
+    const int *eidx;      // optional indirection (nullptr = identity)
+    const int *rowk;      // optional: column = rowk[e] instead of ind[e]
+    const int *colmap;    // column -> position
+};
+void launch_ell_width(int nslices, const int *start, const int *cnt, const int *order, int nrows, const int *ptr,
+                      int *width, hipStream_t s);
+template <typename TS, typename TD>
+void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *order, int nrows, EllSrc src,
+                     const int *off, const int *width, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val,
+                     hipStream_t s);
+// off[c] = 64 * exclusive scan of width (int32); returns via scan kernels.
+void launch_mul64(const int *in, int *out, int n, hipStream_t s);
 
 // ------------------------------------------------------------ PCG
 struct PcgScalars {
@@ -106,22 +137,23 @@ struct PcgScalars {
 enum { kPcgOk = 0, kPcgBreakdown = 1, kPcgTimeout = 2 };
 
 struct SolveArgs {
-    int n;
-    // A (PCG system), CSR == CSC
-    const int *a_ptr;
-    const int *a_ind;
+    int n;                        // rows; index n is a dummy slot holding 0
+    // A (PCG system) in sliced ELL over position tiles
+    int ntiles;
+    const int *a_off, *a_width, *a_ind;
     const double *a_val;
-    // factor
+    // level schedule (positions)
     int nchunks;
-    const int *chunk_start, *chunk_cnt, *order;
-    const int *l_ptr, *l_ind;     // rows of L (forward, pull)
-    const int *lt_ptr, *lt_ind;   // rows of L^T (backward, pull)
-    const void *l_val, *lt_val, *diag;
-    // vectors
+    const int *chunk_start, *chunk_cnt;
+    // L rows (forward) and L^T rows (backward) in sliced ELL per chunk
+    const int *lf_off, *lf_width, *lf_ind;
+    const int *lb_off, *lb_width, *lb_ind;
+    const void *lf_val, *lb_val, *diag;
+    // vectors in position numbering, n + 1 entries
     double *x, *r, *ap, *p0, *p1;
     void *y, *z;                  // precision of the factor, sentinel protocol
     PcgScalars *sc;
-    double *partials;             // >= reduce blocks
+    double *partials;             // >= 2 * reduce blocks
     unsigned int *ctr_fwd, *ctr_bwd;
     int reduce_blocks;
     long long spin_ticks;

@@ -13,12 +13,17 @@
 //             forward sweep y = L^-1 r over level-ordered chunks;
 //   K3 bwd    z = L^-T (D+ y) over chunks in reverse level order;
 //   K4 dot    r'z block partials, last block computes beta and the stop test.
+// All vectors live in the level-order position numbering, so a 64-row chunk of
+// the sweeps (and a 64-row tile of the SpMV) reads and writes its own rows
+// with coalesced accesses; L, L^T and A are stored as sliced ELL (entry t of
+// slice row l at off + 64 t + l), so index/value loads are coalesced too.
+// Padding entries point at the dummy slot n, which always holds zero.
 // The sweeps are persistent: a wave claims the next 64-row chunk of the level
-// order; a row reads each dependency with a plain (cached) load and, only if
-// it still holds the sentinel, polls it with agent-scope loads.  Values are
-// written once per sweep with agent-scope stores, so no fences are needed;
-// each consumer resets the vector it drains back to the sentinel.
-// All reductions use a fixed grid and a fixed summation order.
+// order; a row issues its dependency loads eight at a time with plain (cached)
+// loads and, only for a value still holding the sentinel, polls it with
+// agent-scope loads.  Values are written once per sweep with agent-scope
+// stores, so no fences are needed; each consumer resets the vector it drains
+// back to the sentinel.  All reductions use a fixed grid and a fixed order.
 #include "common.cuh"
 #include "kernels.h"
 
@@ -64,13 +69,11 @@ __device__ __forceinline__ bool last_block(unsigned int *arrive, int *lds_flag)
 }
 
 template <typename T>
-__device__ __forceinline__ T wait_value(T *p, long long spin_ticks, PcgScalars *sc)
+__device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
 {
-    T v = *p;                                   // cached fast path
-    if (!is_sent(v)) return v;
     long long t0 = wall_clock64();
     while (true) {
-        v = ld_relaxed_val(p);
+        T v = ld_relaxed_val(p);
         if (!is_sent(v)) return v;
         if (ld_relaxed(&sc->status) & kPcgTimeout) return (T)0;
         if (wall_clock64() - t0 > spin_ticks) {
@@ -79,6 +82,39 @@ __device__ __forceinline__ T wait_value(T *p, long long spin_ticks, PcgScalars *
         }
         __builtin_amdgcn_s_sleep(1);
     }
+}
+
+// acc - sum_t val[t] * vec[ind[t]] over one sliced-ELL row, in entry order.
+template <typename T>
+__device__ __forceinline__ T ell_row_solve(const int *ind, const T *val, int width, T *vec, T acc, long long spin,
+                                           PcgScalars *sc)
+{
+    constexpr int B = 8;
+    int t = 0;
+    for (; t + B <= width; t += B) {
+        int j[B];
+        T l[B], v[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            j[u] = ind[(t + u) * kWave];
+            l[u] = val[(t + u) * kWave];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) v[u] = vec[j[u]];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (is_sent(v[u])) v[u] = wait_slow(vec + j[u], spin, sc);
+            acc = fma_t(-l[u], v[u], acc);
+        }
+    }
+    for (; t < width; ++t) {
+        int j = ind[t * kWave];
+        T l = val[t * kWave];
+        T v = vec[j];
+        if (is_sent(v)) v = wait_slow(vec + j, spin, sc);
+        acc = fma_t(-l, v, acc);
+    }
+    return acc;
 }
 
 // ---------------------------------------------------------------- K1
@@ -92,18 +128,35 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
     if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
     const double beta = first ? 0.0 : sc->beta;
     const T *z = (const T *)a.z;
+    const int lane = lane_id();
     double acc_dot = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
-        double pi = beta * p_old[i] + (double)z[i];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
+        const int tile = p / kWave;
+        const int w = a.a_width[tile];
+        const int *ind = a.a_ind + a.a_off[tile] + lane;
+        const double *val = a.a_val + a.a_off[tile] + lane;
         double acc = 0.0;
-        for (int e = a.a_ptr[i]; e < a.a_ptr[i + 1]; ++e) {
-            int j = a.a_ind[e];
-            double pj = beta * p_old[j] + (double)z[j];
-            acc = fma_t(a.a_val[e], pj, acc);
+        int t = 0;
+        for (; t + 4 <= w; t += 4) {
+            int j[4];
+            double v[4], pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { j[u] = ind[(t + u) * kWave]; v[u] = val[(t + u) * kWave]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = beta * p_old[j[u]] + (double)z[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
         }
-        p_new[i] = pi;
-        a.ap[i] = acc;
-        acc_dot += pi * acc;
+        for (; t < w; ++t) {
+            int j = ind[t * kWave];
+            acc = fma_t(val[t * kWave], beta * p_old[j] + (double)z[j], acc);
+        }
+        if (p < a.n) {
+            double pi = beta * p_old[p] + (double)z[p];
+            p_new[p] = pi;
+            a.ap[p] = acc;
+            acc_dot += pi * acc;
+        }
     }
     double bs = block_sum(acc_dot, lds);
     if (threadIdx.x == 0) st_relaxed_val(a.partials + blockIdx.x, bs);
@@ -123,14 +176,29 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
     }
 }
 
-// Plain SpMV for timing: y = A x (same gather pattern as K1 with beta = 0).
-__global__ void __launch_bounds__(kRedBlock) k_spmv_plain(int n, const int *ptr, const int *ind, const double *val,
-                                                          const double *x, double *y)
+// Plain SpMV for timing: y = A x over the same sliced-ELL layout.
+__global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const double *x, double *y)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int lane = lane_id();
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
+        const int tile = p / kWave;
+        const int w = a.a_width[tile];
+        const int *ind = a.a_ind + a.a_off[tile] + lane;
+        const double *val = a.a_val + a.a_off[tile] + lane;
         double acc = 0.0;
-        for (int e = ptr[i]; e < ptr[i + 1]; ++e) acc = fma_t(val[e], x[ind[e]], acc);
-        y[i] = acc;
+        int t = 0;
+        for (; t + 4 <= w; t += 4) {
+            int j[4];
+            double v[4], xj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { j[u] = ind[(t + u) * kWave]; v[u] = val[(t + u) * kWave]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xj[u] = x[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = fma_t(v[u], xj[u], acc);
+        }
+        for (; t < w; ++t) acc = fma_t(val[t * kWave], x[ind[t * kWave]], acc);
+        if (p < a.n) y[p] = acc;
     }
 }
 
@@ -141,7 +209,7 @@ __global__ void __launch_bounds__(kWave) k_trsv_fwd(SolveArgs a, const double *p
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const double alpha = update_rx ? sc->alpha : 0.0;
-    const T *lval = (const T *)a.l_val;
+    const T *lval = (const T *)a.lf_val;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
     const int lane = lane_id();
@@ -151,19 +219,20 @@ __global__ void __launch_bounds__(kWave) k_trsv_fwd(SolveArgs a, const double *p
         c = bcast(c, 0);
         if (c >= a.nchunks) break;
         const int cnt = a.chunk_cnt[c];
+        const int k = a.chunk_start[c] + lane;
+        const int off = a.lf_off[c] + lane;
+        const int w = a.lf_width[c];
+        double rk = 0.0;
         if (lane < cnt) {
-            const int k = a.order[a.chunk_start[c] + lane];
-            double rk = a.r[k];
+            rk = a.r[k];
             if (update_rx) {
                 rk = fma_t(-alpha, a.ap[k], rk);
                 a.r[k] = rk;
                 a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
             }
-            T acc = (T)rk;
-            for (int e = a.l_ptr[k]; e < a.l_ptr[k + 1]; ++e) {
-                T v = wait_value(y + a.l_ind[e], a.spin_ticks, sc);
-                acc = fma_t(-lval[e], v, acc);
-            }
+        }
+        T acc = ell_row_solve<T>(a.lf_ind + off, lval + off, w, y, (T)rk, a.spin_ticks, sc);
+        if (lane < cnt) {
             st_relaxed_val(y + k, unsent(acc));
             z[k] = sent_value(T());
         }
@@ -177,7 +246,7 @@ __global__ void __launch_bounds__(kWave) k_trsv_bwd(SolveArgs a)
 {
     PcgScalars *sc = a.sc;
     if (sc->done) return;
-    const T *ltval = (const T *)a.lt_val;
+    const T *ltval = (const T *)a.lb_val;
     const T *diag = (const T *)a.diag;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
@@ -189,19 +258,18 @@ __global__ void __launch_bounds__(kWave) k_trsv_bwd(SolveArgs a)
         if (c >= a.nchunks) break;
         c = a.nchunks - 1 - c;
         const int cnt = a.chunk_cnt[c];
+        const int k = a.chunk_start[c] + lane;
+        const int off = a.lb_off[c] + lane;
+        const int w = a.lb_width[c];
+        T yk = (T)0;
         if (lane < cnt) {
-            const int k = a.order[a.chunk_start[c] + lane];
-            T yk = y[k];
+            yk = y[k];
             y[k] = sent_value(T());
             T dk = diag[k];
             if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
-            T acc = yk;
-            for (int e = a.lt_ptr[k]; e < a.lt_ptr[k + 1]; ++e) {
-                T v = wait_value(z + a.lt_ind[e], a.spin_ticks, sc);
-                acc = fma_t(-ltval[e], v, acc);
-            }
-            st_relaxed_val(z + k, unsent(acc));
         }
+        T acc = ell_row_solve<T>(a.lb_ind + off, ltval + off, w, z, yk, a.spin_ticks, sc);
+        if (lane < cnt) st_relaxed_val(z + k, unsent(acc));
         if (ld_relaxed(&sc->status) & kPcgTimeout) break;
     }
 }
@@ -260,13 +328,20 @@ __global__ void __launch_bounds__(kRedBlock) k_residual(SolveArgs a)
     __shared__ double lds[kRedBlock / kWave];
     __shared__ int flag;
     PcgScalars *sc = a.sc;
+    const int lane = lane_id();
     double acc = 0.0, accb = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
+        const int tile = p / kWave;
+        const int w = a.a_width[tile];
+        const int *ind = a.a_ind + a.a_off[tile] + lane;
+        const double *val = a.a_val + a.a_off[tile] + lane;
         double s = 0.0;
-        for (int e = a.a_ptr[i]; e < a.a_ptr[i + 1]; ++e) s = fma_t(a.a_val[e], a.x[a.a_ind[e]], s);
-        double d = s - a.b[i];
-        acc += d * d;
-        accb += a.b[i] * a.b[i];
+        for (int t = 0; t < w; ++t) s = fma_t(val[t * kWave], a.x[ind[t * kWave]], s);
+        if (p < a.n) {
+            double d = s - a.b[p];
+            acc += d * d;
+            accb += a.b[p] * a.b[p];
+        }
     }
     double bs = block_sum(acc, lds);
     double bb = block_sum(accb, lds);
@@ -291,6 +366,7 @@ __global__ void k_fill_sentinel(T *v, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = sent_value(T());
 }
 
+template <typename T>
 __global__ void k_pcg_prepare(SolveArgs a)
 {
     // r = b, x = 0, p = 0, Ap = 0 (solver.hpp:268-277, 462-466)
@@ -301,7 +377,13 @@ __global__ void k_pcg_prepare(SolveArgs a)
         a.p1[i] = 0.0;
         a.ap[i] = 0.0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.ctr_fwd = 0u;
+        *a.ctr_bwd = 0u;
+        a.x[a.n] = 0.0; a.r[a.n] = 0.0; a.p0[a.n] = 0.0; a.p1[a.n] = 0.0; a.ap[a.n] = 0.0;
+        ((T *)a.y)[a.n] = (T)0;     // dummy slot read by ELL padding
+        ((T *)a.z)[a.n] = (T)0;
+    }
 }
 
 __global__ void k_bnorm(SolveArgs a)
@@ -350,7 +432,7 @@ void launch_pcg_init(const SolveArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.y, a.n);
     hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, a.n);
-    hipLaunchKernelGGL(k_pcg_prepare, dim3(2048), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_pcg_prepare<T>, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bnorm, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
     hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)a.p0, 0);
     hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
@@ -373,8 +455,7 @@ void launch_pcg_iteration(const SolveArgs &a, int parity, hipStream_t s)
 template <typename T>
 void launch_spmv_only(const SolveArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spmv_plain, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a.n, a.a_ptr, a.a_ind, a.a_val,
-                       (const double *)a.p0, a.ap);
+    hipLaunchKernelGGL(k_spmv_plain, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)a.p0, a.ap);
 }
 
 template <typename T>

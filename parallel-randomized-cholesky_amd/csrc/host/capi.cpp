@@ -84,18 +84,24 @@ struct rc_handle_s {
     int64_t nnz_low = 0;
 
     // factor inputs / working set
-    DevVec lo_ptr, lo_row, lo_w;
-    DevVec dep, head, nfill, queue, level, ctr, pool, out_row, out_val, out_start, out_cnt, diag64, scratch;
+    DevVec lo_ptr, lo_row, lo_w, dep_init;
+    DevVec dep, head, nfill, level, list0, list1, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
+        out_cnt, diag64, scratch;
     long long pool_cap = 0, out_cap = 0;
     int scratch_cap = 0;
+    int rounds = 0;
 
-    // factor result (solve numbering)
+    // factor result: CSR of L^T in the factored numbering (export / dump)
     bool factored = false;
     int prec = 64;
-    DevVec lt_ptr, lt_ind, lt_val, l_ptr, l_ind, l_val, diagT, order, chunk_start, chunk_cnt;
-    DevVec tmp_a, tmp_b, tmp_scan;
-    int64_t nnz_lt = 0;
-    int nlevels = 0, nchunks = 0;
+    DevVec lt_ptr, lt_ind, lt_val, diagT;
+    // solve layout: level-order positions, sliced ELL of L, L^T and A
+    DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP;
+    DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
+    DevVec a_off, a_width, a_ell_ind, a_ell_val;
+    DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
+    int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
+    int nlevels = 0, nchunks = 0, ntiles = 0;
     rc_factor_stats fstats{};
 
     // PCG
@@ -388,6 +394,12 @@ static int prepare(rc_handle_t h)
     int rc = lower_part(h->fac, lp, lr, lw);
     if (rc != RC_OK) return fail(h, rc, "lower part too large");
     h->nnz_low = (int64_t)lr.size();
+    // dep[r] = strict-lower entries (r, j), j < r: the reference's count of
+    // strict-upper entries of column r (pre_process.hpp:474-496)
+    std::vector<int> dep((size_t)h->nf, 0);
+    for (int v : lr) dep[(size_t)v]++;
+    RC_HIP(h->dep_init.ensure(dep.size() * sizeof(int)));
+    RC_HIP(hipMemcpyAsync(h->dep_init.p, dep.data(), dep.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
     RC_HIP(h->lo_ptr.ensure(lp.size() * sizeof(int)));
     RC_HIP(h->lo_row.ensure(std::max<size_t>(lr.size(), 1) * sizeof(int)));
     RC_HIP(h->lo_w.ensure(std::max<size_t>(lw.size(), 1) * sizeof(double)));
@@ -448,8 +460,11 @@ static int alloc_factor_work(rc_handle_t h)
     RC_HIP(h->dep.ensure(nf * sizeof(int)));
     RC_HIP(h->head.ensure(nf * sizeof(int)));
     RC_HIP(h->nfill.ensure(nf * sizeof(int)));
-    RC_HIP(h->queue.ensure(nf * sizeof(int)));
     RC_HIP(h->level.ensure(nf * sizeof(int)));
+    RC_HIP(h->list0.ensure(nf * sizeof(int)));
+    RC_HIP(h->list1.ensure(nf * sizeof(int)));
+    RC_HIP(h->round_count.ensure((nf + 2) * sizeof(unsigned int)));
+    RC_HIP(h->wave_state.ensure((size_t)h->num_blocks * 4 * sizeof(long long)));
     RC_HIP(h->out_start.ensure(nf * sizeof(long long)));
     RC_HIP(h->out_cnt.ensure(nf * sizeof(int)));
     RC_HIP(h->diag64.ensure(nf * sizeof(double)));
@@ -461,72 +476,115 @@ static int alloc_factor_work(rc_handle_t h)
     return RC_OK;
 }
 
+static FactorArgs factor_args(rc_handle_t h)
+{
+    FactorArgs a;
+    a.n = h->nf;
+    a.lo_ptr = h->lo_ptr.as<int>();
+    a.lo_row = h->lo_row.as<int>();
+    a.lo_w = h->lo_w.as<double>();
+    a.dep = h->dep.as<int>();
+    a.head = h->head.as<int>();
+    a.nfill = h->nfill.as<int>();
+    a.level = h->level.as<int>();
+    a.list0 = h->list0.as<int>();
+    a.list1 = h->list1.as<int>();
+    a.round_count = h->round_count.as<unsigned int>();
+    a.wave_state = h->wave_state.as<long long>();
+    a.ctr = h->ctr.as<unsigned long long>();
+    a.pool = h->pool.as<FillRec>();
+    a.pool_cap = h->pool_cap;
+    a.out_row = h->out_row.as<int>();
+    a.out_val = h->out_val.as<double>();
+    a.out_cap = h->out_cap;
+    a.out_start = h->out_start.as<long long>();
+    a.out_cnt = h->out_cnt.as<int>();
+    a.diag = h->diag64.as<double>();
+    a.scratch = h->scratch.as<char>();
+    a.scratch_cap = h->scratch_cap;
+    a.seed = h->opt.seed;
+    return a;
+}
+
 static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
 {
+    const long long slack = factor_pool_slack(h->num_blocks);
     if (h->pool_cap == 0) {
-        h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low));
-        h->out_cap = h->pool_cap + h->nnz_low + 16;
+        h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low)) + slack;
+        h->out_cap = h->pool_cap + h->nnz_low + slack;
         h->scratch_cap = 2048;
     }
     *regrowths = 0;
+    hipStream_t s = h->stream;
     for (int attempt = 0; attempt < 8; ++attempt) {
         int rc = alloc_factor_work(h);
         if (rc != RC_OK) return rc;
-        FactorArgs a;
-        a.n = h->nf;
-        a.lo_ptr = h->lo_ptr.as<int>();
-        a.lo_row = h->lo_row.as<int>();
-        a.lo_w = h->lo_w.as<double>();
-        a.dep = h->dep.as<int>();
-        a.head = h->head.as<int>();
-        a.nfill = h->nfill.as<int>();
-        a.queue = h->queue.as<int>();
-        a.level = h->level.as<int>();
-        a.ctr = h->ctr.as<unsigned long long>();
-        a.pool = h->pool.as<FillRec>();
-        a.pool_cap = h->pool_cap;
-        a.out_row = h->out_row.as<int>();
-        a.out_val = h->out_val.as<double>();
-        a.out_cap = h->out_cap;
-        a.out_start = h->out_start.as<long long>();
-        a.out_cnt = h->out_cnt.as<int>();
-        a.diag = h->diag64.as<double>();
-        a.scratch = h->scratch.as<char>();
-        a.scratch_cap = h->scratch_cap;
-        a.seed = h->opt.seed;
-        a.spin_ticks = h->spin_ticks;
-        RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), h->stream));
+        FactorArgs a = factor_args(h);
         hipEvent_t e0, e1;
         RC_HIP(hipEventCreate(&e0));
         RC_HIP(hipEventCreate(&e1));
-        RC_HIP(hipEventRecord(e0, h->stream));
-        launch_factor(a, h->num_blocks, h->stream);
+        RC_HIP(hipEventRecord(e0, s));
+        RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), s));
+        RC_HIP(hipMemsetAsync(h->round_count.p, 0, ((size_t)h->nf + 2) * sizeof(unsigned int), s));
+        RC_HIP(hipMemsetAsync(h->wave_state.p, 0, (size_t)h->num_blocks * 4 * sizeof(long long), s));
+        launch_factor_setup(a, h->dep_init.as<int>(), s);
+        // rounds in growing batches; a round whose list is empty returns at once
+        int round = 0, batch = 8;
+        unsigned long long st = 0;
+        unsigned int *probe = nullptr;
+        RC_HIP(hipHostMalloc((void **)&probe, 4 * sizeof(unsigned int), hipHostMallocDefault));
+        while (true) {
+            int nr = std::min(batch, h->nf + 1 - round);
+            if (nr <= 0) break;
+            launch_factor_rounds(a, h->num_blocks, round, nr, s);
+            round += nr;
+            hipError_t e = hipMemcpyAsync(probe, h->round_count.as<unsigned int>() + round, sizeof(unsigned int),
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(probe + 2, h->ctr.as<unsigned long long>() + kCtrStatus,
+                                                    sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { (void)hipHostFree(probe); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
+            std::memcpy(&st, probe + 2, sizeof(st));
+            if (st != 0 || probe[0] == 0) break;
+            batch = std::min(batch * 2, 256);
+        }
+        (void)hipHostFree(probe);
+        launch_factor_finish(a, s);
         RC_HIP(hipGetLastError());
-        RC_HIP(hipEventRecord(e1, h->stream));
+        RC_HIP(hipEventRecord(e1, s));
         unsigned long long c[kCtrCount];
-        RC_HIP(hipMemcpyAsync(c, h->ctr.p, sizeof(c), hipMemcpyDeviceToHost, h->stream));
-        RC_HIP(hipStreamSynchronize(h->stream));
+        RC_HIP(hipMemcpyAsync(c, h->ctr.p, sizeof(c), hipMemcpyDeviceToHost, s));
+        std::vector<unsigned int> counts((size_t)round + 1);
+        RC_HIP(hipMemcpyAsync(counts.data(), h->round_count.p, counts.size() * sizeof(unsigned int),
+                              hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *ms_out = ms;
-        unsigned long long st = c[kCtrStatus];
+        st = c[kCtrStatus];
         if (st == 0) {
-            if (c[kCtrQHead] < (unsigned long long)h->nf || c[kCtrQTail] != (unsigned long long)h->nf)
-                return fail(h, RC_ERR_STATE, "elimination did not retire every column (queue " +
-                                                 std::to_string(c[kCtrQTail]) + "/" + std::to_string(h->nf) + ")");
+            unsigned long long done = 0;
+            int used = 0;
+            for (size_t i = 0; i < counts.size(); ++i) {
+                done += counts[i];
+                if (counts[i]) used = (int)i + 1;
+            }
+            if (done != (unsigned long long)(h->nf - 1))
+                return fail(h, RC_ERR_STATE, "elimination did not retire every column (" + std::to_string(done) + "/" +
+                                                 std::to_string(h->nf - 1) + ")");
+            h->rounds = used;
             h->fstats.nnz_factor = (int64_t)c[kCtrNnzOff] + h->nf;
             h->fstats.fills = (int64_t)c[kCtrFills];
             h->fstats.max_degree = (int64_t)c[kCtrMaxDeg];
             h->fstats.zero_pivots = (int32_t)c[kCtrZeroPivots];
             return RC_OK;
         }
-        if (st & (kStatusTimeout | kStatusInconsistent))
-            return fail(h, (st & kStatusTimeout) ? RC_ERR_TIMEOUT : RC_ERR_STATE,
-                        "elimination kernel aborted (status " + std::to_string(st) + ")");
-        if (st & kStatusFillPool) h->pool_cap *= 2;
-        if (st & (kStatusOutPool | kStatusFillPool)) h->out_cap = h->pool_cap + h->nnz_low + 16;
+        if (st & kStatusInconsistent)
+            return fail(h, RC_ERR_STATE, "elimination kernel found an inconsistent fill list");
+        if (st & kStatusFillPool) h->pool_cap = 2 * (h->pool_cap - slack) + slack;
+        if (st & (kStatusOutPool | kStatusFillPool)) h->out_cap = h->pool_cap + h->nnz_low + slack;
         if (st & kStatusScratch) h->scratch_cap = (int)std::min<int64_t>(pow2_at_least((int64_t)c[kCtrScratchNeed]), 1 << 28);
         ++*regrowths;
     }
@@ -534,6 +592,13 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
 }
 
 }  // extern "C"
+
+static int bit_width(int64_t v)
+{
+    int b = 0;
+    while (v > 0) { ++b; v >>= 1; }
+    return b;
+}
 
 template <typename T>
 static int assemble(rc_handle_t h, double *ms_out)
@@ -554,8 +619,9 @@ static int assemble(rc_handle_t h, double *ms_out)
     a.out_val = h->out_val.as<double>();
     a.diag_in = h->diag64.as<double>();
     a.level = h->level.as<int>();
-
     const size_t nsz = (size_t)ns + 1;
+
+    // 1. CSR of L^T in the factored numbering (ground row dropped)
     RC_HIP(h->tmp_a.ensure(nsz * sizeof(int)));
     RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(ns + 1) * sizeof(int) + 64));
     RC_HIP(h->lt_ptr.ensure(nsz * sizeof(int)));
@@ -570,20 +636,24 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->diagT.ensure((size_t)std::max(ns, 1) * sizeof(T)));
     launch_lt_fill<T>(a, h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.as<T>(), h->diagT.as<T>(), s);
 
-    // transpose
+    // 2. rows of L: stable radix sort of the entries by row (columns stay ascending)
+    const size_t big = (size_t)std::max(nnz, ns) + 1;
+    RC_HIP(h->keys0.ensure(big * sizeof(uint32_t)));
+    RC_HIP(h->keys1.ensure(big * sizeof(uint32_t)));
+    RC_HIP(h->vals0.ensure(big * sizeof(int)));
+    RC_HIP(h->vals1.ensure(big * sizeof(int)));
+    RC_HIP(h->rowk.ensure(big * sizeof(int)));
     RC_HIP(h->l_ptr.ensure(nsz * sizeof(int)));
-    RC_HIP(h->l_ind.ensure((size_t)std::max(nnz, 1) * sizeof(int)));
-    RC_HIP(h->l_val.ensure((size_t)std::max(nnz, 1) * sizeof(T)));
-    RC_HIP(h->tmp_b.ensure(nsz * sizeof(int)));
-    RC_HIP(hipMemsetAsync(h->tmp_a.p, 0, nsz * sizeof(int), s));
-    launch_count_rows(nnz, h->lt_ind.as<int>(), h->tmp_a.as<int>(), s);
-    scan_exclusive(h->tmp_a.as<int>(), h->l_ptr.as<int>(), ns, h->tmp_scan.as<int>(), s);
-    RC_HIP(hipMemcpyAsync(h->tmp_b.p, h->l_ptr.p, nsz * sizeof(int), hipMemcpyDeviceToDevice, s));
-    launch_transpose_scatter<T>(ns, h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.as<T>(), h->tmp_b.as<int>(),
-                                h->l_ind.as<int>(), h->l_val.as<T>(), s);
-    launch_segmented_sort<T>(ns, h->l_ptr.as<int>(), h->l_ind.as<int>(), h->l_val.as<T>(), s);
+    const size_t sort_tmp = radix_tmp_ints((int)big);
+    RC_HIP(h->tmp_b.ensure(std::max(sort_tmp, nsz * 2) * sizeof(int)));
+    uint32_t *k0 = h->keys0.as<uint32_t>(), *k1 = h->keys1.as<uint32_t>();
+    launch_copy_keys(h->lt_ind.as<int>(), k0, nnz, s);
+    launch_iota(h->vals0.as<int>(), nnz, s);
+    radix_sort_pairs(k0, h->vals0.as<int>(), k1, h->vals1.as<int>(), nnz, bit_width(ns), h->tmp_b.as<int>(), s);
+    launch_segment_bounds(nnz, k0, 0, ns, h->l_ptr.as<int>(), s);
+    launch_row_expand(ns, h->lt_ptr.as<int>(), h->rowk.as<int>(), s);
 
-    // level schedule
+    // 3. level order: rows sorted by (level, descending length of their L row)
     unsigned long long *d_max = h->ctr.as<unsigned long long>() + kCtrCount - 1;
     RC_HIP(hipMemsetAsync(d_max, 0, sizeof(unsigned long long), s));
     launch_level_max(ns, h->level.as<int>(), d_max, s);
@@ -592,20 +662,21 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(hipStreamSynchronize(s));
     const int nl = ns > 0 ? (int)maxlev + 1 : 0;
     h->nlevels = nl;
+    const int lvl_bits = bit_width(nl);
+    const int len_bits = lvl_bits + 16 <= 32 ? 16 : 0;
+    RC_HIP(h->order.ensure(nsz * sizeof(int)));
+    RC_HIP(h->pos_of.ensure(nsz * sizeof(int)));
+    launch_level_keys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, k0, s);
+    launch_iota(h->order.as<int>(), ns, s);
+    radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + len_bits, h->tmp_b.as<int>(), s);
+    launch_inverse_perm(ns, h->order.as<int>(), h->pos_of.as<int>(), s);
     const size_t nlsz = (size_t)nl + 1;
-    RC_HIP(h->tmp_a.ensure(std::max(nsz, nlsz * 2) * sizeof(int)));
-    RC_HIP(h->tmp_b.ensure(std::max(nsz, nlsz * 3) * sizeof(int)));
-    RC_HIP(h->tmp_scan.ensure((scan_tmp_elems(std::max(ns, nl) + 1)) * sizeof(int) + 64));
-    int *lcnt = h->tmp_a.as<int>();
-    int *lptr = h->tmp_b.as<int>();
-    int *lpos = lptr + nlsz;
-    int *cptr = lpos + nlsz;
-    RC_HIP(hipMemsetAsync(lcnt, 0, nlsz * sizeof(int), s));
-    launch_level_count(ns, h->level.as<int>(), lcnt, s);
-    scan_exclusive(lcnt, lptr, nl, h->tmp_scan.as<int>(), s);
-    RC_HIP(hipMemcpyAsync(lpos, lptr, nlsz * sizeof(int), hipMemcpyDeviceToDevice, s));
-    RC_HIP(h->order.ensure((size_t)std::max(ns, 1) * sizeof(int)));
-    launch_level_scatter(ns, h->level.as<int>(), lpos, h->order.as<int>(), s);
+    RC_HIP(h->tmp_a.ensure(std::max(nsz, nlsz * 3) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(ns, nl) + 1) * sizeof(int) + 64));
+    int *lptr = h->tmp_a.as<int>();
+    int *lcnt = lptr + nlsz;
+    int *cptr = lcnt + nlsz;
+    launch_segment_bounds(ns, k0, len_bits, nl, lptr, s);
     launch_chunk_count(nl, lptr, lcnt, s);
     scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
     int nchunks = 0;
@@ -615,6 +686,65 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     launch_chunk_fill(nchunks, nl, lptr, cptr, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
+
+    // 4. sliced ELL of L (forward) and L^T (backward) over the chunks
+    const size_t csz = (size_t)nchunks + 1;
+    RC_HIP(h->lf_width.ensure(csz * sizeof(int)));
+    RC_HIP(h->lf_off.ensure(csz * sizeof(int)));
+    RC_HIP(h->lb_width.ensure(csz * sizeof(int)));
+    RC_HIP(h->lb_off.ensure(csz * sizeof(int)));
+    RC_HIP(h->tmp_a.ensure(std::max(std::max(nsz, nlsz * 3), csz) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(std::max(ns, nl), nchunks) + 1) * sizeof(int) + 64));
+    auto ell_offsets = [&](const int *width, int *off, int nslices, int64_t &total) -> int {
+        launch_mul64(width, h->tmp_a.as<int>(), nslices, s);
+        scan_exclusive(h->tmp_a.as<int>(), off, nslices, h->tmp_scan.as<int>(), s);
+        int t = 0;
+        RC_HIP(hipMemcpyAsync(&t, off + nslices, sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
+        total = t;
+        return RC_OK;
+    };
+    launch_ell_width(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns,
+                     h->l_ptr.as<int>(), h->lf_width.as<int>(), s);
+    launch_ell_width(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns,
+                     h->lt_ptr.as<int>(), h->lb_width.as<int>(), s);
+    int rc = ell_offsets(h->lf_width.as<int>(), h->lf_off.as<int>(), nchunks, h->ell_l);
+    if (rc != RC_OK) return rc;
+    rc = ell_offsets(h->lb_width.as<int>(), h->lb_off.as<int>(), nchunks, h->ell_u);
+    if (rc != RC_OK) return rc;
+    RC_HIP(h->lf_ind.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(int)));
+    RC_HIP(h->lf_val.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(T)));
+    RC_HIP(h->lb_ind.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(int)));
+    RC_HIP(h->lb_val.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(T)));
+    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), h->pos_of.as<int>()};
+    launch_ell_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns, fs,
+                          h->lf_off.as<int>(), h->lf_width.as<int>(), ns, (T)0, h->lf_ind.as<int>(), h->lf_val.as<T>(), s);
+    EllSrc bs{h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.p, nullptr, nullptr, h->pos_of.as<int>()};
+    launch_ell_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns, bs,
+                          h->lb_off.as<int>(), h->lb_width.as<int>(), ns, (T)0, h->lb_ind.as<int>(), h->lb_val.as<T>(), s);
+    RC_HIP(h->diagP.ensure((size_t)std::max(ns, 1) * sizeof(T)));
+    launch_gather_t<T>(h->diagT.as<T>(), h->order.as<int>(), h->diagP.as<T>(), ns, s);
+
+    // 5. A in position numbering, sliced ELL over 64-position tiles
+    const int ntiles = (ns + 63) / 64;
+    h->ntiles = ntiles;
+    const size_t tsz = (size_t)ntiles + 1;
+    RC_HIP(h->a_width.ensure(tsz * sizeof(int)));
+    RC_HIP(h->a_off.ensure(tsz * sizeof(int)));
+    RC_HIP(h->tmp_a.ensure(std::max(std::max(std::max(nsz, nlsz * 3), csz), tsz) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(std::max(std::max(ns, nl), nchunks), ntiles) + 1) * sizeof(int) + 64));
+    launch_ell_width(ntiles, nullptr, nullptr, h->order.as<int>(), ns, h->a_ptr.as<int>(), h->a_width.as<int>(), s);
+    rc = ell_offsets(h->a_width.as<int>(), h->a_off.as<int>(), ntiles, h->ell_a);
+    if (rc != RC_OK) return rc;
+    RC_HIP(h->a_ell_ind.ensure((size_t)std::max<int64_t>(h->ell_a, 1) * sizeof(int)));
+    RC_HIP(h->a_ell_val.ensure((size_t)std::max<int64_t>(h->ell_a, 1) * sizeof(double)));
+    EllSrc as{h->a_ptr.as<int>(), h->a_ind.as<int>(), h->a_val.p, nullptr, nullptr, h->pos_of.as<int>()};
+    launch_ell_fill<double, double>(ntiles, nullptr, nullptr, h->order.as<int>(), ns, as, h->a_off.as<int>(),
+                                    h->a_width.as<int>(), ns, -0.0, h->a_ell_ind.as<int>(), h->a_ell_val.as<double>(), s);
+    // user index of every position (RHS gather / solution scatter)
+    RC_HIP(h->idx_user.ensure(nsz * sizeof(int)));
+    launch_gather_i(h->perm_d.as<int>(), h->order.as<int>(), h->idx_user.as<int>(), ns, s);
+
     RC_HIP(hipGetLastError());
     RC_HIP(hipEventRecord(e1, s));
     RC_HIP(hipEventSynchronize(e1));
@@ -869,20 +999,23 @@ static SolveArgs solve_args(rc_handle_t h)
 {
     SolveArgs a;
     a.n = h->ns;
-    a.a_ptr = h->a_ptr.as<int>();
-    a.a_ind = h->a_ind.as<int>();
-    a.a_val = h->a_val.as<double>();
+    a.ntiles = h->ntiles;
+    a.a_off = h->a_off.as<int>();
+    a.a_width = h->a_width.as<int>();
+    a.a_ind = h->a_ell_ind.as<int>();
+    a.a_val = h->a_ell_val.as<double>();
     a.nchunks = h->nchunks;
     a.chunk_start = h->chunk_start.as<int>();
     a.chunk_cnt = h->chunk_cnt.as<int>();
-    a.order = h->order.as<int>();
-    a.l_ptr = h->l_ptr.as<int>();
-    a.l_ind = h->l_ind.as<int>();
-    a.lt_ptr = h->lt_ptr.as<int>();
-    a.lt_ind = h->lt_ind.as<int>();
-    a.l_val = h->l_val.p;
-    a.lt_val = h->lt_val.p;
-    a.diag = h->diagT.p;
+    a.lf_off = h->lf_off.as<int>();
+    a.lf_width = h->lf_width.as<int>();
+    a.lf_ind = h->lf_ind.as<int>();
+    a.lb_off = h->lb_off.as<int>();
+    a.lb_width = h->lb_width.as<int>();
+    a.lb_ind = h->lb_ind.as<int>();
+    a.lf_val = h->lf_val.p;
+    a.lb_val = h->lb_val.p;
+    a.diag = h->diagP.p;
     a.x = h->x.as<double>();
     a.r = h->r.as<double>();
     a.ap = h->ap.as<double>();
@@ -902,7 +1035,7 @@ static SolveArgs solve_args(rc_handle_t h)
 
 static int alloc_solve(rc_handle_t h)
 {
-    const size_t n = (size_t)std::max(h->ns, 1);
+    const size_t n = (size_t)h->ns + 1;          // + dummy slot
     const size_t s = h->prec == 32 ? 4 : 8;
     const size_t old_y = h->y.p ? (size_t)h->y.p : 0;
     RC_HIP(h->x.ensure(n * 8));
@@ -993,7 +1126,7 @@ static int set_rhs_device(rc_handle_t h, const double *b_user)
     int rc = alloc_solve(h);
     if (rc != RC_OK) return rc;
     RC_HIP(hipMemcpyAsync(h->bu.p, b_user, (size_t)h->ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    launch_gather(h->bu.as<double>(), h->perm_d.as<int>(), h->b.as<double>(), h->ns, h->stream);
+    launch_gather(h->bu.as<double>(), h->idx_user.as<int>(), h->b.as<double>(), h->ns, h->stream);
     RC_HIP(hipGetLastError());
     h->rhs_set = true;
     return RC_OK;
@@ -1072,8 +1205,8 @@ static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
 
 static int fetch_solution(rc_handle_t h, double *x_user)
 {
-    // x_user[perm[i]] = x[i]
-    launch_scatter(h->x.as<double>(), h->perm_d.as<int>(), h->bu.as<double>(), h->ns, h->stream);
+    // x_user[idx_user[p]] = x[p]
+    launch_scatter(h->x.as<double>(), h->idx_user.as<int>(), h->bu.as<double>(), h->ns, h->stream);
     RC_HIP(hipMemcpyAsync(x_user, h->bu.p, (size_t)h->ns * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     RC_HIP(hipStreamSynchronize(h->stream));
     return RC_OK;
