@@ -62,8 +62,9 @@ long long factor_pool_slack(int num_blocks);
 int factor_occupancy_blocks(int device);
 
 // ------------------------------------------------------------ assembly
-// Exclusive scan of n int32 counts into out[0..n] (out[n] = total).
-// tmp needs scan_tmp_elems(n) ints.
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total, so `out`
+// needs n + 1 entries).  tmp needs scan_tmp_elems(n) ints and must not
+// overlap `out`.
 size_t scan_tmp_elems(int n);
 void scan_exclusive(const int *in, int *out, int n, int *tmp, hipStream_t s);
 

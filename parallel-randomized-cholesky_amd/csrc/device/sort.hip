@@ -89,7 +89,7 @@ size_t radix_tmp_ints(int n)
     size_t ntiles = (size_t)(n + kSortTile - 1) / kSortTile;
     if (ntiles < 1) ntiles = 1;
     size_t cnt = ntiles * kDigits;
-    return cnt * 2 + scan_tmp_elems((int)cnt) + 16;
+    return (cnt + 1) * 2 + scan_tmp_elems((int)cnt) + 16;
 }
 
 // Sorts (k0, v0) by key; the result is left in (k0, v0).  k1/v1 are scratch of
@@ -99,9 +99,9 @@ void radix_sort_pairs(uint32_t *k0, int *v0, uint32_t *k1, int *v1, int n, int k
     if (n <= 1) return;
     int ntiles = (n + kSortTile - 1) / kSortTile;
     int cnt = ntiles * kDigits;
-    int *counts = tmp;
-    int *offsets = tmp + cnt;
-    int *scan_tmp = tmp + 2 * cnt;
+    int *counts = tmp;                  // cnt
+    int *offsets = tmp + (cnt + 1);     // cnt + 1: scan_exclusive writes the total at [cnt]
+    int *scan_tmp = tmp + 2 * (cnt + 1);
     int passes = (key_bits + 7) / 8;
     if (passes < 1) passes = 1;
     uint32_t *ka = k0, *kb = k1;

@@ -621,9 +621,13 @@ static int assemble(rc_handle_t h, double *ms_out)
     a.level = h->level.as<int>();
     const size_t nsz = (size_t)ns + 1;
 
+    // Work buffers sized once for the whole assembly (levels <= ns, chunks <= ns,
+    // tiles <= ns/64 + 1), so nothing is reallocated while kernels that read
+    // it may still be in flight.
+    RC_HIP(h->tmp_a.ensure(3 * (nsz + 1) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(ns + 2) * sizeof(int) + 64));
+
     // 1. CSR of L^T in the factored numbering (ground row dropped)
-    RC_HIP(h->tmp_a.ensure(nsz * sizeof(int)));
-    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(ns + 1) * sizeof(int) + 64));
     RC_HIP(h->lt_ptr.ensure(nsz * sizeof(int)));
     launch_lt_counts(a, h->tmp_a.as<int>(), s);
     scan_exclusive(h->tmp_a.as<int>(), h->lt_ptr.as<int>(), ns, h->tmp_scan.as<int>(), s);
@@ -671,8 +675,6 @@ static int assemble(rc_handle_t h, double *ms_out)
     radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + len_bits, h->tmp_b.as<int>(), s);
     launch_inverse_perm(ns, h->order.as<int>(), h->pos_of.as<int>(), s);
     const size_t nlsz = (size_t)nl + 1;
-    RC_HIP(h->tmp_a.ensure(std::max(nsz, nlsz * 3) * sizeof(int)));
-    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(ns, nl) + 1) * sizeof(int) + 64));
     int *lptr = h->tmp_a.as<int>();
     int *lcnt = lptr + nlsz;
     int *cptr = lcnt + nlsz;
@@ -693,8 +695,6 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->lf_off.ensure(csz * sizeof(int)));
     RC_HIP(h->lb_width.ensure(csz * sizeof(int)));
     RC_HIP(h->lb_off.ensure(csz * sizeof(int)));
-    RC_HIP(h->tmp_a.ensure(std::max(std::max(nsz, nlsz * 3), csz) * sizeof(int)));
-    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(std::max(ns, nl), nchunks) + 1) * sizeof(int) + 64));
     auto ell_offsets = [&](const int *width, int *off, int nslices, int64_t &total) -> int {
         launch_mul64(width, h->tmp_a.as<int>(), nslices, s);
         scan_exclusive(h->tmp_a.as<int>(), off, nslices, h->tmp_scan.as<int>(), s);
@@ -731,8 +731,6 @@ static int assemble(rc_handle_t h, double *ms_out)
     const size_t tsz = (size_t)ntiles + 1;
     RC_HIP(h->a_width.ensure(tsz * sizeof(int)));
     RC_HIP(h->a_off.ensure(tsz * sizeof(int)));
-    RC_HIP(h->tmp_a.ensure(std::max(std::max(std::max(nsz, nlsz * 3), csz), tsz) * sizeof(int)));
-    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(std::max(std::max(std::max(ns, nl), nchunks), ntiles) + 1) * sizeof(int) + 64));
     launch_ell_width(ntiles, nullptr, nullptr, h->order.as<int>(), ns, h->a_ptr.as<int>(), h->a_width.as<int>(), s);
     rc = ell_offsets(h->a_width.as<int>(), h->a_off.as<int>(), ntiles, h->ell_a);
     if (rc != RC_OK) return rc;
