@@ -270,6 +270,52 @@ __global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const
         chunk_cnt[c] = (end - start) < kWave ? (end - start) : kWave;
     }
 }
+__global__ void k_sweep_layout(int nchunks, const int *cnt, const int *width, int *slices, int *size)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        int g = sweep_group(cnt[c]);
+        int sl = (width[c] + g - 1) / g;
+        slices[c] = sl;
+        size[c] = sl * sweep_rows_pow2(cnt[c]) * g;
+    }
+}
+
+template <typename TS, typename TD>
+__global__ void k_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src,
+                             const int *off, const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    const TS *sval = (const TS *)src.val;
+    for (int c = wave; c < nchunks; c += nwaves) {
+        const int nc = cnt[c];
+        const int P = sweep_rows_pow2(nc);
+        const int G = sweep_group(nc);
+        const int stride = P * G;
+        const int total = slices[c] * stride;
+        const int o = off[c];
+        for (int idx = lane; idx < total; idx += kWave) {
+            int sl = idx / stride, t = idx - sl * stride;
+            int i = t / G, g = t - i * G;
+            int et = sl * G + g;
+            int ind = pad_ind;
+            TD v = pad_val;
+            if (i < nc) {
+                int row = order[start[c] + i];
+                int b = src.ptr[row], len = src.ptr[row + 1] - b;
+                if (et < len) {
+                    int e = src.eidx ? src.eidx[b + et] : b + et;
+                    int col = src.rowk ? src.rowk[e] : src.ind[e];
+                    ind = src.colmap ? src.colmap[col] : col;
+                    v = (TD)sval[e];
+                }
+            }
+            ell_ind[o + idx] = ind;
+            ell_val[o + idx] = v;
+        }
+    }
+}
 }  // namespace
 
 void launch_lt_counts(const AssembleArgs &a, int *cnt, hipStream_t s)
@@ -354,6 +400,26 @@ void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *
 
 template void launch_lt_fill<float>(const AssembleArgs &, const int *, int *, float *, float *, hipStream_t);
 template void launch_lt_fill<double>(const AssembleArgs &, const int *, int *, double *, double *, hipStream_t);
+void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *slices, int *size, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sweep_layout, dim3(1024), dim3(256), 0, s, nchunks, cnt, width, slices, size);
+}
+
+template <typename TS, typename TD>
+void launch_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src, const int *off,
+                       const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val, hipStream_t s)
+{
+    int g = (nchunks + 3) / 4;
+    if (g > 16384) g = 16384;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL((k_sweep_fill<TS, TD>), dim3(g), dim3(256), 0, s, nchunks, start, cnt, order, src, off, slices,
+                       pad_ind, pad_val, ell_ind, ell_val);
+}
+
+template void launch_sweep_fill<float, float>(int, const int *, const int *, const int *, EllSrc, const int *,
+                                             const int *, int, float, int *, float *, hipStream_t);
+template void launch_sweep_fill<double, double>(int, const int *, const int *, const int *, EllSrc, const int *,
+                                               const int *, int, double, int *, double *, hipStream_t);
 template void launch_gather_t<float>(const float *, const int *, float *, int, hipStream_t);
 template void launch_gather_t<double>(const double *, const int *, double *, int, hipStream_t);
 template void launch_ell_fill<float, float>(int, const int *, const int *, const int *, int, EllSrc, const int *,

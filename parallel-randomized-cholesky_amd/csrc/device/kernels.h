@@ -127,6 +127,28 @@ void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *o
 // off[c] = 64 * exclusive scan of width (int32); returns via scan kernels.
 void launch_mul64(const int *in, int *out, int n, hipStream_t s);
 
+// Grouped sliced ELL for the triangular sweeps: a chunk of cnt <= 64 rows is
+// processed by one 256-thread workgroup with G = min(256 / P, 64) lanes per
+// row (P = next power of two >= cnt).  Entry t of chunk row i lives in slice
+// s = t / G at off[c] + s * P * G + i * G + (t % G).  slices[c] =
+// ceil(width[c] / G); size[c] = slices[c] * P * G.
+constexpr int kSweepThreads = 256;
+__host__ __device__ inline int sweep_rows_pow2(int cnt)
+{
+    int p = 1;
+    while (p < cnt) p <<= 1;
+    return p;
+}
+__host__ __device__ inline int sweep_group(int cnt)
+{
+    int g = kSweepThreads / sweep_rows_pow2(cnt);
+    return g > 64 ? 64 : g;
+}
+void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *slices, int *size, hipStream_t s);
+template <typename TS, typename TD>
+void launch_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src, const int *off,
+                       const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val, hipStream_t s);
+
 // ------------------------------------------------------------ PCG
 struct PcgScalars {
     double rz, rz0, alpha, beta, pap, rr, bb;
@@ -146,7 +168,8 @@ struct SolveArgs {
     // level schedule (positions)
     int nchunks;
     const int *chunk_start, *chunk_cnt;
-    // L rows (forward) and L^T rows (backward) in sliced ELL per chunk
+    // L rows (forward) and L^T rows (backward) in grouped sliced ELL per chunk
+    // (lf_width / lb_width hold the slice counts)
     const int *lf_off, *lf_width, *lf_ind;
     const int *lb_off, *lb_width, *lb_ind;
     const void *lf_val, *lb_val, *diag;
@@ -157,9 +180,11 @@ struct SolveArgs {
     double *partials;             // >= 2 * reduce blocks
     unsigned int *ctr_fwd, *ctr_bwd;
     int reduce_blocks;
+    int sweep_blocks;             // persistent sweep workgroups (occupancy-sized)
     long long spin_ticks;
     const double *b;
 };
+int sweep_occupancy_blocks(int device);
 
 template <typename T> void launch_fill_sentinel(T *v, int n, hipStream_t s);
 template <typename T> void launch_pcg_init(const SolveArgs &a, hipStream_t s);

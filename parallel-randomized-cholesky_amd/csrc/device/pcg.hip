@@ -84,20 +84,22 @@ __device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
     }
 }
 
-// acc - sum_t val[t] * vec[ind[t]] over one sliced-ELL row, in entry order.
+// acc - sum_s val[s] * vec[ind[s]] over this lane's slices of a grouped
+// sliced-ELL row (slice stride `stride`), in slice order.  Dependency loads
+// are issued eight at a time; a value still holding the sentinel is polled.
 template <typename T>
-__device__ __forceinline__ T ell_row_solve(const int *ind, const T *val, int width, T *vec, T acc, long long spin,
-                                           PcgScalars *sc)
+__device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int nsl, int stride, T *vec, T acc,
+                                               long long spin, PcgScalars *sc)
 {
     constexpr int B = 8;
-    int t = 0;
-    for (; t + B <= width; t += B) {
+    int sl = 0;
+    for (; sl + B <= nsl; sl += B) {
         int j[B];
         T l[B], v[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) {
-            j[u] = ind[(t + u) * kWave];
-            l[u] = val[(t + u) * kWave];
+            j[u] = ind[(sl + u) * stride];
+            l[u] = val[(sl + u) * stride];
         }
 #pragma unroll
         for (int u = 0; u < B; ++u) v[u] = vec[j[u]];
@@ -107,14 +109,23 @@ __device__ __forceinline__ T ell_row_solve(const int *ind, const T *val, int wid
             acc = fma_t(-l[u], v[u], acc);
         }
     }
-    for (; t < width; ++t) {
-        int j = ind[t * kWave];
-        T l = val[t * kWave];
+    for (; sl < nsl; ++sl) {
+        int j = ind[sl * stride];
+        T l = val[sl * stride];
         T v = vec[j];
         if (is_sent(v)) v = wait_slow(vec + j, spin, sc);
         acc = fma_t(-l, v, acc);
     }
     return acc;
+}
+
+// Sum over the G lanes of a row group (G a power of two <= 64, groups aligned
+// inside the wave): fixed butterfly, so the result is deterministic.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int G)
+{
+    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
 }
 
 // ---------------------------------------------------------------- K1
@@ -203,74 +214,99 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const dou
 }
 
 // ---------------------------------------------------------------- K2
+// One 256-thread workgroup per claimed chunk; G lanes per row.
 template <typename T>
-__global__ void __launch_bounds__(kWave) k_trsv_fwd(SolveArgs a, const double *p_cur, int update_rx)
+__global__ void __launch_bounds__(kSweepThreads) k_trsv_fwd(SolveArgs a, const double *p_cur, int update_rx)
 {
+    __shared__ int s_chunk, s_stop;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const double alpha = update_rx ? sc->alpha : 0.0;
     const T *lval = (const T *)a.lf_val;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
-    const int lane = lane_id();
+    const int t = threadIdx.x;
     while (true) {
-        int c = 0;
-        if (lane == 0) c = (int)atomicAdd(a.ctr_fwd, 1u);
-        c = bcast(c, 0);
-        if (c >= a.nchunks) break;
+        if (t == 0) {
+            s_chunk = (int)atomicAdd(a.ctr_fwd, 1u);
+            s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
+        }
+        __syncthreads();
+        const int c = s_chunk;
+        const int stop = s_stop;
+        __syncthreads();
+        if (c >= a.nchunks || stop) break;
         const int cnt = a.chunk_cnt[c];
-        const int k = a.chunk_start[c] + lane;
-        const int off = a.lf_off[c] + lane;
-        const int w = a.lf_width[c];
-        double rk = 0.0;
-        if (lane < cnt) {
-            rk = a.r[k];
+        const int G = sweep_group(cnt);
+        const int stride = sweep_rows_pow2(cnt) * G;
+        const int i = t / G, g = t - i * G;
+        const bool active = t < stride && i < cnt;
+        const int k = a.chunk_start[c] + i;
+        T acc = (T)0;
+        if (active && g == 0) {
+            double rk = a.r[k];
             if (update_rx) {
                 rk = fma_t(-alpha, a.ap[k], rk);
                 a.r[k] = rk;
                 a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
             }
+            acc = (T)rk;
         }
-        T acc = ell_row_solve<T>(a.lf_ind + off, lval + off, w, y, (T)rk, a.spin_ticks, sc);
-        if (lane < cnt) {
+        if (t < stride) {
+            const int off = a.lf_off[c] + t;
+            acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, a.lf_width[c], stride, y, acc, a.spin_ticks, sc);
+        }
+        acc = group_sum(acc, G);
+        if (active && g == 0) {
             st_relaxed_val(y + k, unsent(acc));
             z[k] = sent_value(T());
         }
-        if (ld_relaxed(&sc->status) & kPcgTimeout) break;
     }
 }
 
 // ---------------------------------------------------------------- K3
 template <typename T>
-__global__ void __launch_bounds__(kWave) k_trsv_bwd(SolveArgs a)
+__global__ void __launch_bounds__(kSweepThreads) k_trsv_bwd(SolveArgs a)
 {
+    __shared__ int s_chunk, s_stop;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const T *ltval = (const T *)a.lb_val;
     const T *diag = (const T *)a.diag;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
-    const int lane = lane_id();
+    const int t = threadIdx.x;
     while (true) {
-        int c = 0;
-        if (lane == 0) c = (int)atomicAdd(a.ctr_bwd, 1u);
-        c = bcast(c, 0);
-        if (c >= a.nchunks) break;
+        if (t == 0) {
+            s_chunk = (int)atomicAdd(a.ctr_bwd, 1u);
+            s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
+        }
+        __syncthreads();
+        int c = s_chunk;
+        const int stop = s_stop;
+        __syncthreads();
+        if (c >= a.nchunks || stop) break;
         c = a.nchunks - 1 - c;
         const int cnt = a.chunk_cnt[c];
-        const int k = a.chunk_start[c] + lane;
-        const int off = a.lb_off[c] + lane;
-        const int w = a.lb_width[c];
-        T yk = (T)0;
-        if (lane < cnt) {
-            yk = y[k];
+        const int G = sweep_group(cnt);
+        const int stride = sweep_rows_pow2(cnt) * G;
+        const int i = t / G, g = t - i * G;
+        const bool active = t < stride && i < cnt;
+        const int k = a.chunk_start[c] + i;
+        T acc = (T)0;
+        if (active && g == 0) {
+            T yk = y[k];
             y[k] = sent_value(T());
             T dk = diag[k];
             if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
+            acc = yk;
         }
-        T acc = ell_row_solve<T>(a.lb_ind + off, ltval + off, w, z, yk, a.spin_ticks, sc);
-        if (lane < cnt) st_relaxed_val(z + k, unsent(acc));
-        if (ld_relaxed(&sc->status) & kPcgTimeout) break;
+        if (t < stride) {
+            const int off = a.lb_off[c] + t;
+            acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, a.lb_width[c], stride, z, acc, a.spin_ticks, sc);
+        }
+        acc = group_sum(acc, G);
+        if (active && g == 0) st_relaxed_val(z + k, unsent(acc));
     }
 }
 
@@ -415,7 +451,7 @@ __global__ void k_reset_counters(unsigned int *a, unsigned int *b)
 
 int sweep_blocks(const SolveArgs &a)
 {
-    int g = a.nchunks < 16384 ? a.nchunks : 16384;
+    int g = a.nchunks < a.sweep_blocks ? a.nchunks : a.sweep_blocks;
     return g > 0 ? g : 1;
 }
 
@@ -434,8 +470,8 @@ void launch_pcg_init(const SolveArgs &a, hipStream_t s)
     hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, a.n);
     hipLaunchKernelGGL(k_pcg_prepare<T>, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bnorm, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)a.p0, 0);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)a.p0, 0);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
     hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 1);
 }
 
@@ -447,8 +483,8 @@ void launch_pcg_iteration(const SolveArgs &a, int parity, hipStream_t s)
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
     hipLaunchKernelGGL(k_spmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_old, p_new, 0);
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)p_new, 1);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)p_new, 1);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
     hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 0);
 }
 
@@ -461,13 +497,24 @@ void launch_spmv_only(const SolveArgs &a, hipStream_t s)
 template <typename T>
 void launch_trsv_fwd_only(const SolveArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a, (const double *)a.p0, 0);
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)a.p0, 0);
 }
 
 template <typename T>
 void launch_trsv_bwd_only(const SolveArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
+}
+
+int sweep_occupancy_blocks(int device)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trsv_fwd<double>, kSweepThreads, 0) != hipSuccess)
+        per_cu = 4;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
+    if (per_cu < 1) per_cu = 1;
+    return per_cu * cus;
 }
 
 void launch_residual(const SolveArgs &a, hipStream_t s)

@@ -108,6 +108,7 @@ struct rc_handle_s {
     DevVec a_ptr, a_ind, a_val;
     DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d;
     int reduce_blocks = 1024;
+    int sweep_blocks = 1024;
     bool rhs_set = false;
     std::vector<double> xref;
     hipGraphExec_t graph = nullptr;
@@ -215,6 +216,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
         rate_khz = 100000;
     h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
     h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
+    h->sweep_blocks = sweep_occupancy_blocks(o.device);
     *out = h;
     return RC_OK;
 }
@@ -704,24 +706,34 @@ static int assemble(rc_handle_t h, double *ms_out)
         total = t;
         return RC_OK;
     };
-    launch_ell_width(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns,
-                     h->l_ptr.as<int>(), h->lf_width.as<int>(), s);
-    launch_ell_width(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns,
-                     h->lt_ptr.as<int>(), h->lb_width.as<int>(), s);
-    int rc = ell_offsets(h->lf_width.as<int>(), h->lf_off.as<int>(), nchunks, h->ell_l);
+    // grouped layout: width -> slice count in place, slice block sizes -> offsets
+    auto sweep_offsets = [&](const int *ptr, int *slices, int *off, int64_t &total) -> int {
+        launch_ell_width(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns, ptr,
+                         slices, s);
+        launch_sweep_layout(nchunks, h->chunk_cnt.as<int>(), slices, slices, h->tmp_a.as<int>(), s);
+        scan_exclusive(h->tmp_a.as<int>(), off, nchunks, h->tmp_scan.as<int>(), s);
+        int t = 0;
+        RC_HIP(hipMemcpyAsync(&t, off + nchunks, sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
+        total = t;
+        return RC_OK;
+    };
+    int rc = sweep_offsets(h->l_ptr.as<int>(), h->lf_width.as<int>(), h->lf_off.as<int>(), h->ell_l);
     if (rc != RC_OK) return rc;
-    rc = ell_offsets(h->lb_width.as<int>(), h->lb_off.as<int>(), nchunks, h->ell_u);
+    rc = sweep_offsets(h->lt_ptr.as<int>(), h->lb_width.as<int>(), h->lb_off.as<int>(), h->ell_u);
     if (rc != RC_OK) return rc;
     RC_HIP(h->lf_ind.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(int)));
     RC_HIP(h->lf_val.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(T)));
     RC_HIP(h->lb_ind.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(int)));
     RC_HIP(h->lb_val.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(T)));
     EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), h->pos_of.as<int>()};
-    launch_ell_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns, fs,
-                          h->lf_off.as<int>(), h->lf_width.as<int>(), ns, (T)0, h->lf_ind.as<int>(), h->lf_val.as<T>(), s);
+    launch_sweep_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), fs,
+                            h->lf_off.as<int>(), h->lf_width.as<int>(), ns, (T)0, h->lf_ind.as<int>(), h->lf_val.as<T>(),
+                            s);
     EllSrc bs{h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.p, nullptr, nullptr, h->pos_of.as<int>()};
-    launch_ell_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), ns, bs,
-                          h->lb_off.as<int>(), h->lb_width.as<int>(), ns, (T)0, h->lb_ind.as<int>(), h->lb_val.as<T>(), s);
+    launch_sweep_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), bs,
+                            h->lb_off.as<int>(), h->lb_width.as<int>(), ns, (T)0, h->lb_ind.as<int>(), h->lb_val.as<T>(),
+                            s);
     RC_HIP(h->diagP.ensure((size_t)std::max(ns, 1) * sizeof(T)));
     launch_gather_t<T>(h->diagT.as<T>(), h->order.as<int>(), h->diagP.as<T>(), ns, s);
 
@@ -1026,6 +1038,7 @@ static SolveArgs solve_args(rc_handle_t h)
     a.ctr_fwd = h->ctrs.as<unsigned int>();
     a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
     a.reduce_blocks = h->reduce_blocks;
+    a.sweep_blocks = h->sweep_blocks;
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
     return a;
