@@ -133,6 +133,7 @@ void launch_mul64(const int *in, int *out, int n, hipStream_t s);
 // s = t / G at off[c] + s * P * G + i * G + (t % G).  slices[c] =
 // ceil(width[c] / G); size[c] = slices[c] * P * G.
 constexpr int kSweepThreads = 256;
+constexpr int kTailThreads = 1024;          // 4 sweep groups
 __host__ __device__ inline int sweep_rows_pow2(int cnt)
 {
     int p = 1;
@@ -181,6 +182,12 @@ struct SolveArgs {
     unsigned int *ctr_fwd, *ctr_bwd;
     int reduce_blocks;
     int sweep_blocks;             // persistent sweep workgroups (occupancy-sized)
+    // level split: chunks [0, bulk_chunks) belong to the wide levels and run
+    // on the persistent grid; levels [tail_level, nlevels) run in one
+    // workgroup, level by level (chunk range of level L: tail_cptr[L] ..
+    // tail_cptr[L + 1], absolute chunk indices)
+    int bulk_chunks, tail_level, nlevels;
+    const int *tail_cptr;
     long long spin_ticks;
     const double *b;
 };

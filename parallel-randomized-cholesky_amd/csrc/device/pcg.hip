@@ -72,6 +72,7 @@ template <typename T>
 __device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
 {
     long long t0 = wall_clock64();
+    int nap = 1;
     while (true) {
         T v = ld_relaxed_val(p);
         if (!is_sent(v)) return v;
@@ -80,7 +81,9 @@ __device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
             atomicOr(&sc->status, kPcgTimeout);
             return (T)0;
         }
-        __builtin_amdgcn_s_sleep(1);
+        // exponential back-off keeps waiting waves from flooding memory
+        for (int q = 0; q < nap; ++q) __builtin_amdgcn_s_sleep(2);
+        nap = nap < 32 ? nap * 2 : 32;
     }
 }
 
@@ -213,100 +216,119 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const dou
     }
 }
 
-// ---------------------------------------------------------------- K2
-// One 256-thread workgroup per claimed chunk; G lanes per row.
+// ---------------------------------------------------------------- K2/K3
+// Chunk c of the forward sweep, processed by one 256-thread group (thread t
+// of the group); G lanes per row.
 template <typename T>
-__global__ void __launch_bounds__(kSweepThreads) k_trsv_fwd(SolveArgs a, const double *p_cur, int update_rx)
+__device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int c, int t, double alpha, const double *p_cur,
+                                          int update_rx)
 {
-    __shared__ int s_chunk, s_stop;
-    PcgScalars *sc = a.sc;
-    if (sc->done) return;
-    const double alpha = update_rx ? sc->alpha : 0.0;
     const T *lval = (const T *)a.lf_val;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
-    const int t = threadIdx.x;
-    while (true) {
-        if (t == 0) {
-            s_chunk = (int)atomicAdd(a.ctr_fwd, 1u);
-            s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
+    const int cnt = a.chunk_cnt[c];
+    const int G = sweep_group(cnt);
+    const int stride = sweep_rows_pow2(cnt) * G;
+    const int i = t / G, g = t - i * G;
+    const bool active = t < stride && i < cnt;
+    const int k = a.chunk_start[c] + i;
+    T acc = (T)0;
+    if (active && g == 0) {
+        double rk = a.r[k];
+        if (update_rx) {
+            rk = fma_t(-alpha, a.ap[k], rk);
+            a.r[k] = rk;
+            a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
         }
-        __syncthreads();
-        const int c = s_chunk;
-        const int stop = s_stop;
-        __syncthreads();
-        if (c >= a.nchunks || stop) break;
-        const int cnt = a.chunk_cnt[c];
-        const int G = sweep_group(cnt);
-        const int stride = sweep_rows_pow2(cnt) * G;
-        const int i = t / G, g = t - i * G;
-        const bool active = t < stride && i < cnt;
-        const int k = a.chunk_start[c] + i;
-        T acc = (T)0;
-        if (active && g == 0) {
-            double rk = a.r[k];
-            if (update_rx) {
-                rk = fma_t(-alpha, a.ap[k], rk);
-                a.r[k] = rk;
-                a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
-            }
-            acc = (T)rk;
-        }
-        if (t < stride) {
-            const int off = a.lf_off[c] + t;
-            acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, a.lf_width[c], stride, y, acc, a.spin_ticks, sc);
-        }
-        acc = group_sum(acc, G);
-        if (active && g == 0) {
-            st_relaxed_val(y + k, unsent(acc));
-            z[k] = sent_value(T());
-        }
+        acc = (T)rk;
+    }
+    if (t < stride) {
+        const int off = a.lf_off[c] + t;
+        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, a.lf_width[c], stride, y, acc, a.spin_ticks, a.sc);
+    }
+    acc = group_sum(acc, G);
+    if (active && g == 0) {
+        st_relaxed_val(y + k, unsent(acc));
+        z[k] = sent_value(T());
     }
 }
 
-// ---------------------------------------------------------------- K3
 template <typename T>
-__global__ void __launch_bounds__(kSweepThreads) k_trsv_bwd(SolveArgs a)
+__device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int c, int t)
 {
-    __shared__ int s_chunk, s_stop;
-    PcgScalars *sc = a.sc;
-    if (sc->done) return;
     const T *ltval = (const T *)a.lb_val;
     const T *diag = (const T *)a.diag;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
+    const int cnt = a.chunk_cnt[c];
+    const int G = sweep_group(cnt);
+    const int stride = sweep_rows_pow2(cnt) * G;
+    const int i = t / G, g = t - i * G;
+    const bool active = t < stride && i < cnt;
+    const int k = a.chunk_start[c] + i;
+    T acc = (T)0;
+    if (active && g == 0) {
+        T yk = y[k];
+        y[k] = sent_value(T());
+        T dk = diag[k];
+        if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
+        acc = yk;
+    }
+    if (t < stride) {
+        const int off = a.lb_off[c] + t;
+        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, a.lb_width[c], stride, z, acc, a.spin_ticks, a.sc);
+    }
+    acc = group_sum(acc, G);
+    if (active && g == 0) st_relaxed_val(z + k, unsent(acc));
+}
+
+// Persistent part: workgroups claim chunks of the wide levels in order
+// (forward) or reverse order (backward); dependencies are awaited per value.
+template <typename T, bool kFwd>
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const double *p_cur, int update_rx)
+{
+    __shared__ int s_chunk, s_stop;
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
+    unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
     const int t = threadIdx.x;
     while (true) {
         if (t == 0) {
-            s_chunk = (int)atomicAdd(a.ctr_bwd, 1u);
+            s_chunk = (int)atomicAdd(ctr, 1u);
             s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
         }
         __syncthreads();
         int c = s_chunk;
         const int stop = s_stop;
         __syncthreads();
-        if (c >= a.nchunks || stop) break;
-        c = a.nchunks - 1 - c;
-        const int cnt = a.chunk_cnt[c];
-        const int G = sweep_group(cnt);
-        const int stride = sweep_rows_pow2(cnt) * G;
-        const int i = t / G, g = t - i * G;
-        const bool active = t < stride && i < cnt;
-        const int k = a.chunk_start[c] + i;
-        T acc = (T)0;
-        if (active && g == 0) {
-            T yk = y[k];
-            y[k] = sent_value(T());
-            T dk = diag[k];
-            if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
-            acc = yk;
+        if (c >= a.bulk_chunks || stop) break;
+        if (kFwd) fwd_chunk<T>(a, c, t, alpha, p_cur, update_rx);
+        else bwd_chunk<T>(a, a.bulk_chunks - 1 - c, t);
+    }
+}
+
+// Tail part: the narrow levels, one 1024-thread workgroup, level by level;
+// its four 256-thread groups share the chunks of a level, and a barrier
+// separates levels, so nothing is polled.
+template <typename T, bool kFwd>
+__global__ void __launch_bounds__(kTailThreads) k_sweep_tail(SolveArgs a, const double *p_cur, int update_rx)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
+    const int grp = threadIdx.x / kSweepThreads;
+    const int t = threadIdx.x - grp * kSweepThreads;
+    const int ngrp = kTailThreads / kSweepThreads;
+    const int ntail = a.nlevels - a.tail_level;
+    for (int q = 0; q < ntail; ++q) {
+        const int L = kFwd ? a.tail_level + q : a.nlevels - 1 - q;
+        const int c0 = a.tail_cptr[L], c1 = a.tail_cptr[L + 1];
+        for (int c = c0 + grp; c < c1; c += ngrp) {
+            if (kFwd) fwd_chunk<T>(a, c, t, alpha, p_cur, update_rx);
+            else bwd_chunk<T>(a, c, t);
         }
-        if (t < stride) {
-            const int off = a.lb_off[c] + t;
-            acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, a.lb_width[c], stride, z, acc, a.spin_ticks, sc);
-        }
-        acc = group_sum(acc, G);
-        if (active && g == 0) st_relaxed_val(z + k, unsent(acc));
+        __syncthreads();
     }
 }
 
@@ -451,8 +473,28 @@ __global__ void k_reset_counters(unsigned int *a, unsigned int *b)
 
 int sweep_blocks(const SolveArgs &a)
 {
-    int g = a.nchunks < a.sweep_blocks ? a.nchunks : a.sweep_blocks;
+    int g = a.bulk_chunks < a.sweep_blocks ? a.bulk_chunks : a.sweep_blocks;
     return g > 0 ? g : 1;
+}
+
+template <typename T>
+void launch_fwd(const SolveArgs &a, const double *p_cur, int update_rx, hipStream_t s)
+{
+    if (a.bulk_chunks > 0)
+        hipLaunchKernelGGL((k_sweep_bulk<T, true>), dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, p_cur,
+                           update_rx);
+    if (a.tail_level < a.nlevels)
+        hipLaunchKernelGGL((k_sweep_tail<T, true>), dim3(1), dim3(kTailThreads), 0, s, a, p_cur, update_rx);
+}
+
+template <typename T>
+void launch_bwd(const SolveArgs &a, hipStream_t s)
+{
+    if (a.tail_level < a.nlevels)
+        hipLaunchKernelGGL((k_sweep_tail<T, false>), dim3(1), dim3(kTailThreads), 0, s, a, (const double *)nullptr, 0);
+    if (a.bulk_chunks > 0)
+        hipLaunchKernelGGL((k_sweep_bulk<T, false>), dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a,
+                           (const double *)nullptr, 0);
 }
 
 }  // namespace
@@ -470,8 +512,8 @@ void launch_pcg_init(const SolveArgs &a, hipStream_t s)
     hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, a.n);
     hipLaunchKernelGGL(k_pcg_prepare<T>, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bnorm, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)a.p0, 0);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
+    launch_fwd<T>(a, (const double *)a.p0, 0, s);
+    launch_bwd<T>(a, s);
     hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 1);
 }
 
@@ -483,8 +525,8 @@ void launch_pcg_iteration(const SolveArgs &a, int parity, hipStream_t s)
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
     hipLaunchKernelGGL(k_spmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_old, p_new, 0);
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)p_new, 1);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
+    launch_fwd<T>(a, (const double *)p_new, 1, s);
+    launch_bwd<T>(a, s);
     hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 0);
 }
 
@@ -497,19 +539,19 @@ void launch_spmv_only(const SolveArgs &a, hipStream_t s)
 template <typename T>
 void launch_trsv_fwd_only(const SolveArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, (const double *)a.p0, 0);
+    launch_fwd<T>(a, (const double *)a.p0, 0, s);
 }
 
 template <typename T>
 void launch_trsv_bwd_only(const SolveArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a);
+    launch_bwd<T>(a, s);
 }
 
 int sweep_occupancy_blocks(int device)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trsv_fwd<double>, kSweepThreads, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_bulk<double, true>, kSweepThreads, 0) != hipSuccess)
         per_cu = 4;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;

@@ -102,6 +102,8 @@ struct rc_handle_s {
     DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
+    int tail_level = 0, bulk_chunks = 0;
+    DevVec tail_cptr;
     rc_factor_stats fstats{};
 
     // PCG
@@ -690,6 +692,21 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     launch_chunk_fill(nchunks, nl, lptr, cptr, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
+    // level split: the trailing run of levels with fewer than kTailRows rows
+    // goes to the single-workgroup tail sweep
+    {
+        std::vector<int> hl(nlsz), hc(nlsz);
+        RC_HIP(hipMemcpyAsync(hl.data(), lptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipMemcpyAsync(hc.data(), cptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
+        const int kTailRows = 256;
+        int tl = nl;
+        while (tl > 0 && hl[(size_t)tl] - hl[(size_t)tl - 1] < kTailRows) --tl;
+        h->tail_level = tl;
+        h->bulk_chunks = nl > 0 ? hc[(size_t)tl] : 0;
+        RC_HIP(h->tail_cptr.ensure(nlsz * sizeof(int)));
+        RC_HIP(hipMemcpyAsync(h->tail_cptr.p, cptr, nlsz * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
 
     // 4. sliced ELL of L (forward) and L^T (backward) over the chunks
     const size_t csz = (size_t)nchunks + 1;
@@ -1039,6 +1056,10 @@ static SolveArgs solve_args(rc_handle_t h)
     a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
     a.reduce_blocks = h->reduce_blocks;
     a.sweep_blocks = h->sweep_blocks;
+    a.bulk_chunks = h->bulk_chunks;
+    a.tail_level = h->tail_level;
+    a.nlevels = h->nlevels;
+    a.tail_cptr = h->tail_cptr.as<int>();
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
     return a;
