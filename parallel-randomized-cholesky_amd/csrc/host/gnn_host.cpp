@@ -13,6 +13,4 @@ int gnn_order(const Csc &, int, int, const std::vector<float> &, uint64_t, std::
     return RC_ERR_STATE;
 }
 
-std::vector<int> amd_perm(const Csc &, int) { return {}; }
-
 }  // namespace rc

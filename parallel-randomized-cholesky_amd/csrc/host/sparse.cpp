@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <queue>
+#include <utility>
 
 #include "../../../include/rchol.h"
 #include "philox.h"
@@ -230,6 +232,108 @@ std::vector<int> nnz_sort_perm(const Csc &a, int nvert, uint64_t seed, int count
     std::vector<int> perm((size_t)nvert);
     for (int i = 0; i < nvert; ++i) perm[(size_t)i] = p1[(size_t)idx[(size_t)i]];
     return perm;
+}
+
+// Approximate minimum degree on the quotient graph (Amestoy, Davis, Duff,
+// SIAM J. Matrix Anal. Appl. 17(4), 1996): variables keep their remaining
+// variable neighbours A_i and adjacent elements E_i; eliminating the variable
+// p of least approximate degree forms the element L_p = A_p + union of the
+// elements in E_p (which are absorbed), and every i in L_p gets the bound
+//   d_i = min(n - k - 1, d_i + |L_p \ i|, |A_i \ i| + |L_p \ i| + sum_{e in E_i, e != p} |L_e \ L_p|)
+// where |L_e \ L_p| comes from one scan over L_p.  Ties are broken by the
+// smaller vertex index, so the ordering is deterministic.  Supervariable
+// detection and aggressive absorption are not implemented.
+std::vector<int> amd_perm(const Csc &a, int nvert)
+{
+    const int n = nvert;
+    std::vector<std::vector<int>> A((size_t)n), E((size_t)n), Le((size_t)n);
+    for (int j = 0; j < n; ++j)
+        for (int64_t p = a.ptr[(size_t)j]; p < a.ptr[(size_t)j + 1]; ++p) {
+            int i = a.ind[(size_t)p];
+            if (i != j && i < n) A[(size_t)j].push_back(i);
+        }
+    for (auto &v : A) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    std::vector<char> elem((size_t)n, 0), dead((size_t)n, 0);
+    std::vector<int> deg((size_t)n), mark((size_t)n, -1), wmark((size_t)n, -1), w((size_t)n, 0);
+    using QE = std::pair<int, int>;
+    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+    for (int i = 0; i < n; ++i) {
+        deg[(size_t)i] = (int)A[(size_t)i].size();
+        pq.push({deg[(size_t)i], i});
+    }
+    std::vector<int> order;
+    order.reserve((size_t)n);
+    std::vector<int> Lp;
+    for (int k = 0; k < n; ++k) {
+        int p = -1;
+        while (!pq.empty()) {
+            QE t = pq.top();
+            pq.pop();
+            if (!elem[(size_t)t.second] && t.first == deg[(size_t)t.second]) { p = t.second; break; }
+        }
+        if (p < 0) break;
+        // element L_p
+        Lp.clear();
+        mark[(size_t)p] = k;
+        for (int v : A[(size_t)p])
+            if (!elem[(size_t)v] && mark[(size_t)v] != k) { mark[(size_t)v] = k; Lp.push_back(v); }
+        for (int e : E[(size_t)p]) {
+            if (dead[(size_t)e]) continue;
+            for (int v : Le[(size_t)e])
+                if (!elem[(size_t)v] && mark[(size_t)v] != k) { mark[(size_t)v] = k; Lp.push_back(v); }
+            dead[(size_t)e] = 1;                      // absorbed into p
+            std::vector<int>().swap(Le[(size_t)e]);
+        }
+        order.push_back(p);
+        elem[(size_t)p] = 1;
+        std::vector<int>().swap(A[(size_t)p]);
+        std::vector<int>().swap(E[(size_t)p]);
+        Le[(size_t)p] = Lp;
+        // |L_e \ L_p| for every live element adjacent to L_p
+        for (int i : Lp) {
+            auto &Ei = E[(size_t)i];
+            size_t q = 0;
+            for (size_t t = 0; t < Ei.size(); ++t) {
+                int e = Ei[t];
+                if (dead[(size_t)e]) continue;
+                Ei[q++] = e;
+                if (wmark[(size_t)e] != k) {
+                    auto &L = Le[(size_t)e];
+                    size_t r = 0;
+                    for (size_t u = 0; u < L.size(); ++u)
+                        if (!elem[(size_t)L[u]]) L[r++] = L[u];
+                    L.resize(r);
+                    wmark[(size_t)e] = k;
+                    w[(size_t)e] = (int)L.size();
+                }
+                w[(size_t)e] -= 1;
+            }
+            Ei.resize(q);
+        }
+        const int lp = (int)Lp.size();
+        for (int i : Lp) {
+            auto &Ai = A[(size_t)i];
+            size_t q = 0;
+            for (size_t t = 0; t < Ai.size(); ++t) {
+                int v = Ai[t];
+                if (v == p || elem[(size_t)v] || mark[(size_t)v] == k) continue;
+                Ai[q++] = v;
+            }
+            Ai.resize(q);
+            long long ext = (long long)Ai.size() + (lp - 1);
+            for (int e : E[(size_t)i]) ext += w[(size_t)e];
+            E[(size_t)i].push_back(p);
+            long long d = std::min<long long>((long long)(n - k - 1), (long long)deg[(size_t)i] + (lp - 1));
+            d = std::min(d, ext);
+            deg[(size_t)i] = (int)std::max<long long>(d, 0);
+            pq.push({deg[(size_t)i], i});
+        }
+    }
+    if ((int)order.size() != n) return {};
+    return order;
 }
 
 int lower_part(const Csc &a, std::vector<int> &ptr, std::vector<int> &row, std::vector<double> &w)

@@ -162,3 +162,78 @@ def test_out_of_order_calls(rc):
     with pytest.raises(rc.RcholError) as e:
         s.order(rc.ORDER_NNZ_SORT, 1)
     assert e.value.status == rc.ERR_STATE
+
+
+# ------------------------------------------------------------------ AMD ordering
+def _symbolic_fill(n, ptr, ind):
+    """nnz of the strict lower Cholesky factor of the pattern (dense symbolic)."""
+    M = np.zeros((n, n), dtype=bool)
+    for j in range(n):
+        M[ind[ptr[j]:ptr[j + 1]], j] = True
+    cnt = 0
+    for k in range(n):
+        nb = np.where(M[k + 1:, k])[0] + k + 1
+        cnt += len(nb)
+        if len(nb):
+            M[np.ix_(nb, nb)] = True
+    return cnt
+
+
+def _exact_min_degree_fill(n, edges):
+    """Fill of exact minimum degree (ties by smallest index) by brute force."""
+    adj = [set() for _ in range(n)]
+    for u, v in edges:
+        adj[u].add(v); adj[v].add(u)
+    alive = set(range(n))
+    fill = 0
+    while alive:
+        p = min(alive, key=lambda v: (len(adj[v] & alive), v))
+        nb = adj[p] & alive
+        fill += len(nb)
+        for a in nb:
+            adj[a] |= nb - {a}
+        alive.remove(p)
+    return fill
+
+
+@pytest.mark.parametrize("dims", [(12, 12), (7, 6, 5), (30, 9)])
+def test_amd_valid_and_reduces_fill(rc, wl, dims):
+    n, r, c, v = wl.grid_laplacian(dims)
+    s = _host(rc)
+    s.set_triplets(n, r, c, v)
+    nat = _symbolic_fill(*s.prepared_csc()[:3])
+    s.order(rc.ORDER_AMD, 1)
+    p = s.permutation()
+    assert sorted(p.tolist()) == list(range(n))
+    amd = _symbolic_fill(*s.prepared_csc()[:3])
+    assert amd < 0.75 * nat
+    # within 30 % of SuperLU's multiple minimum degree on A + A^T
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    A = sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsc() + 10.0 * sp.eye(n, format="csc")
+    perm_c = sla.splu(A, permc_spec="MMD_AT_PLUS_A").perm_c
+    B = sp.csc_matrix(A[perm_c][:, perm_c])
+    B.sort_indices()
+    mmd = _symbolic_fill(n, B.indptr, B.indices)
+    assert amd <= 1.3 * mmd
+
+
+def test_amd_close_to_exact_minimum_degree_on_small_graphs(rc):
+    rng = np.random.default_rng(3)
+    for trial in range(25):
+        n = int(rng.integers(5, 10))
+        edges = set()
+        for _ in range(int(rng.integers(n, 3 * n))):
+            u, w = rng.integers(0, n, 2)
+            if u != w:
+                edges.add((min(u, w), max(u, w)))
+        edges = sorted(edges)
+        if not edges:
+            continue
+        u = np.array([e[0] for e in edges]); w_ = np.array([e[1] for e in edges])
+        s = _host(rc)
+        s.set_triplets(n, np.concatenate([u, w_]), np.concatenate([w_, u]), -np.ones(2 * len(edges)))
+        s.order(rc.ORDER_AMD, 1)
+        amd = _symbolic_fill(*s.prepared_csc()[:3])
+        md = _exact_min_degree_fill(n, edges)
+        assert amd <= md + 2, (n, edges, amd, md)
