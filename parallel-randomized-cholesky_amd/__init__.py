@@ -37,7 +37,7 @@ EXPORTS = (
     "rc_load_mtx", "rc_set_triplets", "rc_get_size", "rc_order", "rc_get_permutation",
     "rc_set_gnn_weights", "rc_factor", "rc_get_factor", "rc_dump_factor", "rc_load_factor",
     "rc_solve", "rc_make_rhs", "rc_set_reference_solution", "rc_set_rhs", "rc_solve_resident",
-    "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx",
+    "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
 )
 
 
@@ -123,6 +123,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_time_kernel.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]
     lib.rc_get_prepared_csc.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64), vp, vp, vp]
     lib.rc_write_prepared_mtx.argtypes = [vp, ctypes.c_char_p]
+    lib.rc_solve_multi.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, P(SolveStats)]
     _lib = lib
     return lib
 
@@ -261,6 +262,18 @@ class Solver:
         if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
             self._check(rc)
         return x, st.as_dict()
+
+    def solve_multi(self, B, tol=1e-6, allow_breakdown=False):
+        """k <= 64 right-hand sides (columns of B, shape (n, k)); returns X, stats list."""
+        B = np.asarray(B, dtype=np.float64)
+        n, k = B.shape
+        Bc = np.ascontiguousarray(B.T)          # column-major for the C ABI
+        Xc = np.empty_like(Bc)
+        st = (SolveStats * k)()
+        rc = self.lib.rc_solve_multi(self.h, k, _ptr(Bc), _ptr(Xc), float(tol), st)
+        if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
+            self._check(rc)
+        return Xc.T.copy(), [st[q].as_dict() for q in range(k)]
 
     def set_rhs(self, b):
         self._b = np.ascontiguousarray(b, dtype=np.float64)

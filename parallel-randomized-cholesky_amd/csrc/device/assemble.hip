@@ -316,6 +316,29 @@ __global__ void k_sweep_fill(int nchunks, const int *start, const int *cnt, cons
         }
     }
 }
+__global__ void k_pos_csr_count(int n, const int *order, const int *src_ptr, int *cnt)
+{
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        int row = order[p];
+        cnt[p] = src_ptr[row + 1] - src_ptr[row];
+    }
+}
+
+template <typename TS, typename TD>
+__global__ void k_pos_csr_fill(int n, const int *order, EllSrc src, const int *ptr, int *ind, TD *val)
+{
+    const TS *sval = (const TS *)src.val;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        int row = order[p];
+        int b = src.ptr[row], len = src.ptr[row + 1] - b, o = ptr[p];
+        for (int t = 0; t < len; ++t) {
+            int e = src.eidx ? src.eidx[b + t] : b + t;
+            int col = src.rowk ? src.rowk[e] : src.ind[e];
+            ind[o + t] = src.colmap ? src.colmap[col] : col;
+            val[o + t] = (TD)sval[e];
+        }
+    }
+}
 }  // namespace
 
 void launch_lt_counts(const AssembleArgs &a, int *cnt, hipStream_t s)
@@ -420,6 +443,18 @@ template void launch_sweep_fill<float, float>(int, const int *, const int *, con
                                              const int *, int, float, int *, float *, hipStream_t);
 template void launch_sweep_fill<double, double>(int, const int *, const int *, const int *, EllSrc, const int *,
                                                const int *, int, double, int *, double *, hipStream_t);
+void launch_pos_csr_count(int n, const int *order, const int *src_ptr, int *cnt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pos_csr_count, dim3(2048), dim3(256), 0, s, n, order, src_ptr, cnt);
+}
+template <typename TS, typename TD>
+void launch_pos_csr_fill(int n, const int *order, EllSrc src, const int *ptr, int *ind, TD *val, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pos_csr_fill<TS, TD>), dim3(2048), dim3(256), 0, s, n, order, src, ptr, ind, val);
+}
+template void launch_pos_csr_fill<float, float>(int, const int *, EllSrc, const int *, int *, float *, hipStream_t);
+template void launch_pos_csr_fill<double, double>(int, const int *, EllSrc, const int *, int *, double *, hipStream_t);
+
 template void launch_gather_t<float>(const float *, const int *, float *, int, hipStream_t);
 template void launch_gather_t<double>(const double *, const int *, double *, int, hipStream_t);
 template void launch_ell_fill<float, float>(int, const int *, const int *, const int *, int, EllSrc, const int *,

@@ -204,4 +204,44 @@ void launch_gather(const double *src, const int *idx, double *dst, int n, hipStr
 void launch_scatter(const double *src, const int *idx, double *dst, int n, hipStream_t s);
 void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s);
 
+// ------------------------------------------------------------ multi-RHS PCG
+constexpr int kMaxRhs = 64;
+struct MultiScalars {
+    double rz[kMaxRhs], rz0[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs], bb[kMaxRhs], res2[kMaxRhs];
+    int iter[kMaxRhs], done[kMaxRhs], breakdown[kMaxRhs];
+    double tol;
+    int maxit, stop_rule, status, all_done;
+    unsigned int arrive_spmv, arrive_dot, arrive_res, pad;
+};
+
+struct MultiArgs {
+    int n, K, k;                  // rows, lane stride (pow2 >= k), right-hand sides
+    const int *a_ptr, *a_ind;     // CSR in position numbering
+    const double *a_val;
+    const int *l_ptr, *l_ind, *lt_ptr, *lt_ind;
+    const void *l_val, *lt_val, *diag;
+    int nchunks;
+    const int *chunk_start, *chunk_cnt;
+    double *x, *r, *ap, *p0, *p1;
+    void *y, *z;
+    const double *b;
+    MultiScalars *sc;
+    double *partials;             // 2 * reduce_blocks * K
+    unsigned int *ctr_fwd, *ctr_bwd;
+    int reduce_blocks, sweep_blocks;
+    long long spin_ticks;
+};
+
+template <typename T> void launch_multi_init(const MultiArgs &a, hipStream_t s);
+template <typename T> void launch_multi_iteration(const MultiArgs &a, int parity, hipStream_t s);
+void launch_multi_residual(const MultiArgs &a, hipStream_t s);
+void launch_multi_gather(const double *B, const int *idx_user, int n, int k, int K, double *b, hipStream_t s);
+void launch_multi_scatter(const double *x, const int *idx_user, int n, int k, int K, double *X, hipStream_t s);
+
+// CSR in position numbering: row p = source row order[p], columns mapped by
+// src.colmap, values converted to TD.
+void launch_pos_csr_count(int n, const int *order, const int *src_ptr, int *cnt, hipStream_t s);
+template <typename TS, typename TD>
+void launch_pos_csr_fill(int n, const int *order, EllSrc src, const int *ptr, int *ind, TD *val, hipStream_t s);
+
 }  // namespace rc

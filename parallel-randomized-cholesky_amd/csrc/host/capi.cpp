@@ -116,6 +116,11 @@ struct rc_handle_s {
     hipGraphExec_t graph = nullptr;
     int graph_prec = 0;
 
+    // multi-RHS: CSRs in position numbering (built lazily per factor) + vectors
+    bool multi_ready = false;
+    DevVec m_aptr, m_aind, m_aval, m_lptr, m_lind, m_lval, m_ltptr, m_ltind, m_ltval;
+    DevVec m_x, m_r, m_ap, m_p0, m_p1, m_y, m_z, m_b, m_B, m_sc, m_part;
+
     ~rc_handle_s()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -607,6 +612,7 @@ static int bit_width(int64_t v)
 template <typename T>
 static int assemble(rc_handle_t h, double *ms_out)
 {
+    h->multi_ready = false;
     hipStream_t s = h->stream;
     hipEvent_t e0, e1;
     RC_HIP(hipEventCreate(&e0));
@@ -1283,6 +1289,169 @@ int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_sta
     if (rc2 != RC_OK) return rc2;
     if (!h->xref.empty()) sp->err_vs_oracle = a_norm_err(h, x);
     return rc;
+}
+
+// ------------------------------------------------------------------ multi-RHS
+}  // extern "C"
+
+template <typename T>
+static int build_multi(rc_handle_t h)
+{
+    if (h->multi_ready) return RC_OK;
+    hipStream_t s = h->stream;
+    const int ns = h->ns;
+    const size_t nsz = (size_t)ns + 1;
+    auto pos_csr = [&](const int *src_ptr, EllSrc src, DevVec &ptr, DevVec &ind, DevVec &val, size_t vbytes) -> int {
+        RC_HIP(ptr.ensure(nsz * sizeof(int)));
+        launch_pos_csr_count(ns, h->order.as<int>(), src_ptr, h->tmp_a.as<int>(), s);
+        scan_exclusive(h->tmp_a.as<int>(), ptr.as<int>(), ns, h->tmp_scan.as<int>(), s);
+        int nnz = 0;
+        RC_HIP(hipMemcpyAsync(&nnz, ptr.as<int>() + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
+        RC_HIP(ind.ensure((size_t)std::max(nnz, 1) * sizeof(int)));
+        RC_HIP(val.ensure((size_t)std::max(nnz, 1) * vbytes));
+        return RC_OK;
+    };
+    EllSrc as{h->a_ptr.as<int>(), h->a_ind.as<int>(), h->a_val.p, nullptr, nullptr, h->pos_of.as<int>()};
+    int rc = pos_csr(h->a_ptr.as<int>(), as, h->m_aptr, h->m_aind, h->m_aval, sizeof(double));
+    if (rc != RC_OK) return rc;
+    launch_pos_csr_fill<double, double>(ns, h->order.as<int>(), as, h->m_aptr.as<int>(), h->m_aind.as<int>(),
+                                        h->m_aval.as<double>(), s);
+    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), h->pos_of.as<int>()};
+    rc = pos_csr(h->l_ptr.as<int>(), fs, h->m_lptr, h->m_lind, h->m_lval, sizeof(T));
+    if (rc != RC_OK) return rc;
+    launch_pos_csr_fill<T, T>(ns, h->order.as<int>(), fs, h->m_lptr.as<int>(), h->m_lind.as<int>(),
+                              h->m_lval.as<T>(), s);
+    EllSrc bs{h->lt_ptr.as<int>(), h->lt_ind.as<int>(), h->lt_val.p, nullptr, nullptr, h->pos_of.as<int>()};
+    rc = pos_csr(h->lt_ptr.as<int>(), bs, h->m_ltptr, h->m_ltind, h->m_ltval, sizeof(T));
+    if (rc != RC_OK) return rc;
+    launch_pos_csr_fill<T, T>(ns, h->order.as<int>(), bs, h->m_ltptr.as<int>(), h->m_ltind.as<int>(),
+                              h->m_ltval.as<T>(), s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipStreamSynchronize(s));
+    h->multi_ready = true;
+    return RC_OK;
+}
+
+extern "C" {
+
+int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (k <= 0 || k > kMaxRhs || !B || !X || !(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
+    if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_solve_multi");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = h->prec == 32 ? build_multi<float>(h) : build_multi<double>(h);
+    if (rc != RC_OK) return rc;
+    hipStream_t s = h->stream;
+    int K = 1;
+    while (K < k) K <<= 1;
+    const size_t n = (size_t)h->ns;
+    const size_t tot = (n + 1) * (size_t)K;
+    const size_t sz = h->prec == 32 ? 4 : 8;
+    RC_HIP(h->m_x.ensure(tot * 8));
+    RC_HIP(h->m_r.ensure(tot * 8));
+    RC_HIP(h->m_ap.ensure(tot * 8));
+    RC_HIP(h->m_p0.ensure(tot * 8));
+    RC_HIP(h->m_p1.ensure(tot * 8));
+    RC_HIP(h->m_b.ensure(tot * 8));
+    RC_HIP(h->m_y.ensure(tot * sz));
+    RC_HIP(h->m_z.ensure(tot * sz));
+    RC_HIP(h->m_B.ensure(n * (size_t)k * 8));
+    RC_HIP(h->m_sc.ensure(sizeof(MultiScalars)));
+    RC_HIP(h->m_part.ensure((size_t)h->reduce_blocks * 2 * K * sizeof(double)));
+    RC_HIP(h->ctrs.ensure(4 * sizeof(unsigned int)));
+    MultiArgs a;
+    a.n = (int)n;
+    a.K = K;
+    a.k = k;
+    a.a_ptr = h->m_aptr.as<int>();
+    a.a_ind = h->m_aind.as<int>();
+    a.a_val = h->m_aval.as<double>();
+    a.l_ptr = h->m_lptr.as<int>();
+    a.l_ind = h->m_lind.as<int>();
+    a.lt_ptr = h->m_ltptr.as<int>();
+    a.lt_ind = h->m_ltind.as<int>();
+    a.l_val = h->m_lval.p;
+    a.lt_val = h->m_ltval.p;
+    a.diag = h->diagP.p;
+    a.nchunks = h->nchunks;
+    a.chunk_start = h->chunk_start.as<int>();
+    a.chunk_cnt = h->chunk_cnt.as<int>();
+    a.x = h->m_x.as<double>();
+    a.r = h->m_r.as<double>();
+    a.ap = h->m_ap.as<double>();
+    a.p0 = h->m_p0.as<double>();
+    a.p1 = h->m_p1.as<double>();
+    a.y = h->m_y.p;
+    a.z = h->m_z.p;
+    a.b = h->m_b.as<double>();
+    a.sc = h->m_sc.as<MultiScalars>();
+    a.partials = h->m_part.as<double>();
+    a.ctr_fwd = h->ctrs.as<unsigned int>();
+    a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
+    a.reduce_blocks = h->reduce_blocks;
+    a.sweep_blocks = h->sweep_blocks;
+    a.spin_ticks = h->spin_ticks;
+
+    MultiScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.tol = tol;
+    sc.maxit = h->opt.maxit;
+    sc.stop_rule = h->opt.stop_rule;
+    RC_HIP(hipMemcpyAsync(h->m_B.p, B, n * (size_t)k * 8, hipMemcpyHostToDevice, s));
+    RC_HIP(hipMemcpyAsync(h->m_sc.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
+    launch_multi_gather(h->m_B.as<double>(), h->idx_user.as<int>(), (int)n, k, K, h->m_b.as<double>(), s);
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    RC_HIP(hipEventRecord(e0, s));
+    if (h->prec == 32) launch_multi_init<float>(a, s); else launch_multi_init<double>(a, s);
+    int *done_h = nullptr;
+    RC_HIP(hipHostMalloc((void **)&done_h, sizeof(int) * 2, hipHostMallocDefault));
+    done_h[0] = 0;
+    int it = 0;
+    while (it < h->opt.maxit + 2) {
+        for (int t = 0; t < 8; ++t, ++it) {
+            if (h->prec == 32) launch_multi_iteration<float>(a, it & 1, s);
+            else launch_multi_iteration<double>(a, it & 1, s);
+        }
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(done_h, (char *)h->m_sc.p + offsetof(MultiScalars, all_done),
+                                                sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
+        if (done_h[0]) break;
+    }
+    (void)hipHostFree(done_h);
+    RC_HIP(hipEventRecord(e1, s));
+    launch_multi_residual(a, s);
+    RC_HIP(hipMemcpyAsync(&sc, h->m_sc.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+    launch_multi_scatter(a.x, h->idx_user.as<int>(), (int)n, k, K, h->m_B.as<double>(), s);
+    RC_HIP(hipMemcpyAsync(X, h->m_B.p, n * (size_t)k * 8, hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int any_break = 0;
+    for (int q = 0; q < k; ++q) {
+        any_break |= sc.breakdown[q];
+        if (!st) continue;
+        st[q].iterations = sc.iter[q];
+        bool met = h->opt.stop_rule == RC_STOP_REFERENCE ? (sc.rz0[q] <= tol || std::sqrt(sc.rz[q]) / std::sqrt(sc.rz0[q]) <= tol)
+                                                          : true;
+        st[q].converged = (sc.done[q] && !sc.breakdown[q] && (sc.iter[q] < h->opt.maxit || met)) ? 1 : 0;
+        st[q].rel_res_true = sc.bb[q] > 0.0 ? std::sqrt(sc.res2[q]) / std::sqrt(sc.bb[q]) : 0.0;
+        st[q].rel_res_precond = sc.rz0[q] > 0.0 ? std::sqrt(std::max(sc.rz[q], 0.0)) / std::sqrt(sc.rz0[q]) : 0.0;
+        st[q].pcg_ms = ms;
+        st[q].setup_ms = 0.0;
+        st[q].err_vs_oracle = -1.0;
+    }
+    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
+    if (any_break) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown in at least one right-hand side");
+    return RC_OK;
 }
 
 // ------------------------------------------------------------------ timing

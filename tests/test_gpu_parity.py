@@ -187,3 +187,24 @@ def test_resident_solve_and_kernel_timing(rc, wl):
     for k in (rc.KERNEL_SPMV, rc.KERNEL_TRSV_FWD, rc.KERNEL_TRSV_BWD, rc.KERNEL_PCG_ITER):
         ms, by = s.time_kernel(k, reps=3)
         assert ms > 0 and by > 0
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("k", [1, 5, 64])
+def test_multi_rhs_matches_oracle(rc, orc, wl, k, prec):
+    """k right-hand sides in one launch sequence: every column is its own PCG."""
+    say(f"[gpu] multi-rhs k={k} prec={prec}")
+    s, data, kind = _solver(rc, wl, "grid3d_20_log", prec)
+    s.factor()
+    B = np.stack([s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42 + q) for q in range(k)], axis=1)
+    if k == 5:
+        B[:, 3] = 0.0                      # a zero RHS stops before the loop and stays frozen
+    X, sts = s.solve_multi(B, tol=1e-6)
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    tol = 1e-7 if prec == 64 else 1e-4
+    for q in range(k):
+        xo, so = orc.pcg(sys_, f, B[perm, q], tol=1e-6, precision=prec)
+        assert abs(sts[q]["iterations"] - so["iterations"]) <= 1, (q, sts[q], so)
+        assert np.linalg.norm(X[perm, q] - xo) <= tol * max(np.linalg.norm(xo), 1e-300)
+    if k == 5:
+        assert sts[3]["iterations"] == 0 and np.all(X[:, 3] == 0.0)
