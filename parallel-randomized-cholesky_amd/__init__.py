@@ -38,6 +38,7 @@ EXPORTS = (
     "rc_set_gnn_weights", "rc_factor", "rc_get_factor", "rc_dump_factor", "rc_load_factor",
     "rc_solve", "rc_make_rhs", "rc_set_reference_solution", "rc_set_rhs", "rc_solve_resident",
     "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
+    "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
 )
 
 
@@ -124,6 +125,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_get_prepared_csc.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int64), vp, vp, vp]
     lib.rc_write_prepared_mtx.argtypes = [vp, ctypes.c_char_p]
     lib.rc_solve_multi.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, P(SolveStats)]
+    lib.rc_set_permutation.argtypes = [vp, vp]
+    lib.rc_gnn_weight_count.argtypes = []
+    lib.rc_gnn_weight_count.restype = ctypes.c_int64
+    lib.rc_gnn_scores.argtypes = [vp, vp, vp]
+    lib.rc_gnn_gradient.argtypes = [vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -211,6 +217,24 @@ class Solver:
         p = np.empty(n, dtype=np.int32)
         self._check(self.lib.rc_get_permutation(self.h, _ptr(p)))
         return p
+
+    def set_permutation(self, perm):
+        p = np.ascontiguousarray(perm, dtype=np.int32)
+        self._check(self.lib.rc_set_permutation(self.h, _ptr(p)))
+
+    def gnn_scores(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        n, _ = self.size()
+        s = np.empty(n, dtype=np.float32)
+        self._check(self.lib.rc_gnn_scores(self.h, _ptr(w), _ptr(s)))
+        return s
+
+    def gnn_gradient(self, w, ds):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        ds = np.ascontiguousarray(ds, dtype=np.float32)
+        dw = np.empty_like(w)
+        self._check(self.lib.rc_gnn_gradient(self.h, _ptr(w), _ptr(ds), _ptr(dw)))
+        return dw
 
     def set_gnn_weights(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)

@@ -204,6 +204,33 @@ void launch_gather(const double *src, const int *idx, double *dst, int n, hipStr
 void launch_scatter(const double *src, const int *idx, double *dst, int n, hipStream_t s);
 void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s);
 
+// ------------------------------------------------------------ ordering network
+// Weight vector layout (float32): per layer Ws (64 x in), Wn (64 x in), b (64)
+// for in = 2, 64, 64, then wo (64) and bo (1).
+constexpr int kGnnHidden = 64;
+constexpr int kGnnWs1 = 0, kGnnWn1 = 128, kGnnB1 = 256;
+constexpr int kGnnWs2 = 320, kGnnWn2 = 320 + 4096, kGnnB2 = 320 + 8192;
+constexpr int kGnnWs3 = kGnnB2 + 64, kGnnWn3 = kGnnWs3 + 4096, kGnnB3 = kGnnWs3 + 8192;
+constexpr int kGnnWo = kGnnB3 + 64, kGnnBo = kGnnWo + 64;
+constexpr int kGnnParams = kGnnBo + 1;   // 16897
+
+struct GnnArgs {
+    int n;
+    const int *ptr, *ind;         // graph CSR (off-diagonal entries)
+    const float *a;               // w / wdeg(row)        (forward aggregation)
+    const float *at;              // w / wdeg(neighbour)  (transposed aggregation)
+    float *H0, *M1;               // n x 2
+    float *H[3], *M[3], *Z[3];    // n x 64 (index 0..2 = layers 1..3; M[0] unused)
+    float *dA, *dB, *dC;          // n x 64 backward scratch
+    float *part;                  // nblocks x (2*64*64 + 64)
+    int nblocks;
+};
+// keep = true stores every layer (for the backward pass); otherwise layers
+// ping-pong through H[0..1], M[0], Z[0].
+void gnn_forward(const GnnArgs &g, const float *w, bool keep, float *scores, hipStream_t s);
+// dw = d(sum_v ds[v] s[v]) / dw, using the stored forward pass.
+void gnn_backward(const GnnArgs &g, const float *w, const float *ds, float *dw, hipStream_t s);
+
 // ------------------------------------------------------------ multi-RHS PCG
 constexpr int kMaxRhs = 64;
 struct MultiScalars {

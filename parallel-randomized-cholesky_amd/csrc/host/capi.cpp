@@ -116,6 +116,11 @@ struct rc_handle_s {
     hipGraphExec_t graph = nullptr;
     int graph_prec = 0;
 
+    // ordering network: graph + activations of the loaded system
+    bool gnn_ready = false, gnn_keep = false;
+    int gnn_n = 0;
+    DevVec g_ptr, g_ind, g_a, g_at, g_H0, g_M1, g_H[3], g_M[3], g_Z[3], g_dA, g_dB, g_dC, g_part, g_w, g_dw, g_s, g_ds;
+
     // multi-RHS: CSRs in position numbering (built lazily per factor) + vectors
     bool multi_ready = false;
     DevVec m_aptr, m_aind, m_aval, m_lptr, m_lind, m_lval, m_ltptr, m_ltind, m_ltval;
@@ -246,6 +251,7 @@ const char *rc_last_error(rc_handle_t h)
 // ------------------------------------------------------------------ input
 static void reset_system(rc_handle_t h)
 {
+    h->gnn_ready = false;
     h->host_ready = false;
     h->prepared = false;
     h->factored = false;
@@ -321,10 +327,12 @@ int rc_order(rc_handle_t h, int method, uint64_t seed)
         if ((int)p.size() != nv) return fail(h, RC_ERR_ARG, "AMD ordering failed");
         break;
     case RC_ORDER_LEARNED:
-        if (h->gnn_w.empty()) p = nnz_sort_perm(h->base, nv, seed, skip);      // fallback
+        if (h->gnn_w.empty() || h->host_only) p = nnz_sort_perm(h->base, nv, seed, skip);      // fallback
         else {
-            int rc = gnn_order(h->base, nv, skip, h->gnn_w, seed, p, h->err);
+            std::vector<float> sc((size_t)nv);
+            int rc = rc_gnn_scores(h, h->gnn_w.data(), sc.data());
             if (rc != RC_OK) return rc;
+            p = order_from_scores(sc, seed);
         }
         break;
     default: return fail(h, RC_ERR_ARG, "unknown ordering");
@@ -352,6 +360,135 @@ int rc_set_gnn_weights(rc_handle_t h, const float *w, int64_t count)
     if (count < 0 || (count > 0 && !w)) return fail(h, RC_ERR_ARG, "bad weights");
     if (count > 0 && count != gnn_weight_count()) return fail(h, RC_ERR_ARG, "weight count mismatch");
     h->gnn_w.assign(w, w + count);
+    return RC_OK;
+}
+
+int rc_set_permutation(rc_handle_t h, const int *perm)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (!perm) return fail(h, RC_ERR_ARG, "null permutation");
+    const int nv = h->n_user;
+    std::vector<char> seen((size_t)nv, 0);
+    for (int i = 0; i < nv; ++i) {
+        if (perm[i] < 0 || perm[i] >= nv || seen[(size_t)perm[i]]) return fail(h, RC_ERR_ARG, "not a permutation");
+        seen[(size_t)perm[i]] = 1;
+    }
+    h->perm.assign(perm, perm + nv);
+    h->host_ready = false;
+    h->prepared = false;
+    h->factored = false;
+    h->rhs_set = false;
+    invalidate_graph(h);
+    return RC_OK;
+}
+
+int64_t rc_gnn_weight_count(void) { return gnn_weight_count(); }
+
+static int gnn_setup(rc_handle_t h, bool keep)
+{
+    if (h->gnn_ready && (h->gnn_keep || !keep)) return RC_OK;
+    GnnGraph g;
+    gnn_graph(h->base, h->n_user, g);
+    const size_t n = (size_t)g.n, m = std::max<size_t>(g.ind.size(), 1);
+    hipStream_t s = h->stream;
+    RC_HIP(h->g_ptr.ensure((n + 1) * sizeof(int)));
+    RC_HIP(h->g_ind.ensure(m * sizeof(int)));
+    RC_HIP(h->g_a.ensure(m * sizeof(float)));
+    RC_HIP(h->g_at.ensure(m * sizeof(float)));
+    RC_HIP(h->g_H0.ensure(n * 2 * sizeof(float) + 16));
+    RC_HIP(h->g_M1.ensure(n * 2 * sizeof(float) + 16));
+    const int nl = keep ? 3 : 2;
+    for (int l = 0; l < nl; ++l) RC_HIP(h->g_H[l].ensure(n * kGnnHidden * sizeof(float) + 16));
+    for (int l = 0; l < (keep ? 3 : 1); ++l) {
+        RC_HIP(h->g_M[l].ensure(n * kGnnHidden * sizeof(float) + 16));
+        RC_HIP(h->g_Z[l].ensure(n * kGnnHidden * sizeof(float) + 16));
+    }
+    if (keep) {
+        RC_HIP(h->g_dA.ensure(n * kGnnHidden * sizeof(float) + 16));
+        RC_HIP(h->g_dB.ensure(n * kGnnHidden * sizeof(float) + 16));
+        RC_HIP(h->g_dC.ensure(n * kGnnHidden * sizeof(float) + 16));
+        RC_HIP(h->g_part.ensure((size_t)256 * (2 * kGnnHidden * kGnnHidden + kGnnHidden) * sizeof(float)));
+        RC_HIP(h->g_dw.ensure((size_t)kGnnParams * sizeof(float)));
+        RC_HIP(h->g_ds.ensure(n * sizeof(float) + 16));
+    }
+    RC_HIP(h->g_w.ensure((size_t)kGnnParams * sizeof(float)));
+    RC_HIP(h->g_s.ensure(n * sizeof(float) + 16));
+    RC_HIP(hipMemcpyAsync(h->g_ptr.p, g.ptr.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!g.ind.empty()) {
+        RC_HIP(hipMemcpyAsync(h->g_ind.p, g.ind.data(), g.ind.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->g_a.p, g.a.data(), g.a.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->g_at.p, g.at.data(), g.at.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    RC_HIP(hipMemcpyAsync(h->g_H0.p, g.x0.data(), n * 2 * sizeof(float), hipMemcpyHostToDevice, s));
+    RC_HIP(hipStreamSynchronize(s));
+    h->gnn_n = g.n;
+    h->gnn_ready = true;
+    h->gnn_keep = keep;
+    return RC_OK;
+}
+
+static GnnArgs gnn_args(rc_handle_t h)
+{
+    GnnArgs g;
+    g.n = h->gnn_n;
+    g.ptr = h->g_ptr.as<int>();
+    g.ind = h->g_ind.as<int>();
+    g.a = h->g_a.as<float>();
+    g.at = h->g_at.as<float>();
+    g.H0 = h->g_H0.as<float>();
+    g.M1 = h->g_M1.as<float>();
+    for (int l = 0; l < 3; ++l) {
+        g.H[l] = h->g_H[l].as<float>();
+        g.M[l] = h->g_M[l].as<float>();
+        g.Z[l] = h->g_Z[l].as<float>();
+    }
+    g.dA = h->g_dA.as<float>();
+    g.dB = h->g_dB.as<float>();
+    g.dC = h->g_dC.as<float>();
+    g.part = h->g_part.as<float>();
+    g.nblocks = 256;
+    return g;
+}
+
+int rc_gnn_scores(rc_handle_t h, const float *w, float *scores)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (!w || !scores) return fail(h, RC_ERR_ARG, "null array");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = gnn_setup(h, false);
+    if (rc != RC_OK) return rc;
+    hipStream_t s = h->stream;
+    RC_HIP(hipMemcpyAsync(h->g_w.p, w, (size_t)kGnnParams * sizeof(float), hipMemcpyHostToDevice, s));
+    GnnArgs g = gnn_args(h);
+    gnn_forward(g, h->g_w.as<float>(), h->gnn_keep, h->g_s.as<float>(), s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipMemcpyAsync(scores, h->g_s.p, (size_t)h->gnn_n * sizeof(float), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    return RC_OK;
+}
+
+int rc_gnn_gradient(rc_handle_t h, const float *w, const float *dscores, float *dw)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (!w || !dscores || !dw) return fail(h, RC_ERR_ARG, "null array");
+    RC_HIP(hipSetDevice(h->opt.device));
+    int rc = gnn_setup(h, true);
+    if (rc != RC_OK) return rc;
+    hipStream_t s = h->stream;
+    RC_HIP(hipMemcpyAsync(h->g_w.p, w, (size_t)kGnnParams * sizeof(float), hipMemcpyHostToDevice, s));
+    RC_HIP(hipMemcpyAsync(h->g_ds.p, dscores, (size_t)h->gnn_n * sizeof(float), hipMemcpyHostToDevice, s));
+    GnnArgs g = gnn_args(h);
+    gnn_forward(g, h->g_w.as<float>(), true, h->g_s.as<float>(), s);
+    RC_HIP(hipMemsetAsync(h->g_dw.p, 0, (size_t)kGnnParams * sizeof(float), s));
+    gnn_backward(g, h->g_w.as<float>(), h->g_ds.as<float>(), h->g_dw.as<float>(), s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipMemcpyAsync(dw, h->g_dw.p, (size_t)kGnnParams * sizeof(float), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
     return RC_OK;
 }
 

@@ -1,4 +1,5 @@
-// gnn_host.hpp -- learned elimination ordering (graph network) host entry.
+// gnn_host.hpp -- learned elimination ordering: graph features and ordering
+// from network scores (the network kernels are in csrc/device/gnn.hip).
 #pragma once
 
 #include <stdint.h>
@@ -10,12 +11,19 @@
 
 namespace rc {
 
-// Number of float32 parameters of the ordering network.
 int64_t gnn_weight_count();
 
-// Ordering of the first nvert vertices of `a` (ground `skip` excluded from
-// the features) by the network's per-vertex scores.
-int gnn_order(const Csc &a, int nvert, int skip, const std::vector<float> &w, uint64_t seed, std::vector<int> &perm,
-              std::string &err);
+// Graph of the first nvert vertices of `a` (off-diagonal entries, |value| as
+// weight): CSR, forward / transposed aggregation weights, centred inputs.
+struct GnnGraph {
+    int n = 0;
+    std::vector<int> ptr, ind;
+    std::vector<float> a, at, x0;   // x0: n x 2
+};
+void gnn_graph(const Csc &a, int nvert, GnnGraph &g);
+
+// Elimination order from scores: descending score, ties broken by a seeded
+// random permutation (so equal scores fall back to a random order).
+std::vector<int> order_from_scores(const std::vector<float> &scores, uint64_t seed);
 
 }  // namespace rc

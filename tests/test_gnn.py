@@ -1,0 +1,107 @@
+"""Ordering network: the CPU oracle's backward pass against finite
+differences (CPU), and the GPU forward/backward kernels against the oracle
+(GPU)."""
+import numpy as np
+import pytest
+
+import gnn_oracle as go
+
+
+def _small_graph(wl, seed=1):
+    n, r, c, v = wl.grid_laplacian((9, 8), "loguniform", seed=seed)
+    return n, r, c, v
+
+
+def test_weight_layout_matches_library(rc):
+    assert rc.load_library().rc_gnn_weight_count() == go.NPARAM == 16897
+
+
+def test_oracle_backward_matches_finite_differences(wl):
+    n, r, c, v = _small_graph(wl)
+    A, x0 = go.graph(n, r, c, v)
+    w = go.init_weights(3).astype(np.float64)
+    rng = np.random.default_rng(0)
+    ds = rng.standard_normal(n)
+    s, cache = go.forward(A, x0, w, bf16_round=False)
+    dw = go.backward(A, w, cache, ds, bf16_round=False)
+    loss = lambda ww: float(ds @ go.forward(A, x0, ww, bf16_round=False)[0])
+    idx = np.concatenate([rng.choice(go.NPARAM, 40, replace=False), [go.BO, go.WO + 3, 5, 130, 300, 9000]])
+    for i in idx:
+        h = 1e-6
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        # float32 weight path: evaluate in float64 by bypassing the float32 cast
+        fd = (loss_f64(A, x0, wp, ds) - loss_f64(A, x0, wm, ds)) / (2 * h)
+        assert abs(fd - dw[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, dw[i])
+
+
+def loss_f64(A, x0, w, ds):
+    """Exact float64 network (no float32 cast of the weights) for finite differences."""
+    Hp = x0
+    for layer in range(3):
+        o, on, ob, din = go.LAYOUT[layer]
+        Ws, Wn, b = w[o:o + 64 * din].reshape(64, din), w[on:on + 64 * din].reshape(64, din), w[ob:ob + 64]
+        M = A @ Hp
+        Hp = np.maximum(Hp @ Ws.T + M @ Wn.T + b, 0.0)
+    return float(ds @ (Hp @ w[go.WO:go.WO + 64] + w[go.BO]))
+
+
+def test_oracle_backward_catches_a_dropped_transpose(wl):
+    """Planted mistake: A instead of A^T in the message backward fails the check."""
+    n, r, c, v = _small_graph(wl)
+    A, x0 = go.graph(n, r, c, v)
+    w = go.init_weights(4).astype(np.float64)
+    ds = np.random.default_rng(1).standard_normal(n)
+    s, cache = go.forward(A, x0, w, bf16_round=False)
+    good = go.backward(A, w, cache, ds, bf16_round=False)
+    bad = go.backward(A.T.tocsr(), w, cache, ds, bf16_round=False)   # aggregation with the wrong operator
+    i = go.LAYOUT[1][0] + 7
+    h = 1e-6
+    wp, wm = w.copy(), w.copy()
+    wp[i] += h
+    wm[i] -= h
+    fd = (loss_f64(A, x0, wp, ds) - loss_f64(A, x0, wm, ds)) / (2 * h)
+    assert abs(fd - good[i]) <= 1e-5 * max(1.0, abs(fd))
+    assert not np.allclose(good, bad)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(9, 8), (12, 11, 10)])
+def test_gpu_forward_backward_match_oracle(rc, wl, dims):
+    print(f"[gpu] gnn forward/backward dims={dims}", flush=True)
+    n, r, c, v = wl.grid_laplacian(dims, "loguniform", seed=2)
+    s = rc.Solver(device=0)
+    s.set_triplets(n, r, c, v)
+    w = go.init_weights(5)
+    A, x0 = go.graph(n, r, c, v)
+    so, cache = go.forward(A, x0, w)
+    sg = s.gnn_scores(w).astype(np.float64)
+    assert np.max(np.abs(sg - so)) <= 2e-3 * max(1.0, np.max(np.abs(so)))
+    ds = np.random.default_rng(3).standard_normal(n)
+    dwo = go.backward(A, w, cache, ds)
+    dwg = s.gnn_gradient(w, ds).astype(np.float64)
+    err = np.abs(dwg - dwo)
+    scale = np.maximum(np.abs(dwo), 1e-3 * np.max(np.abs(dwo)))
+    assert np.quantile(err / scale, 0.99) < 2e-2
+    assert np.linalg.norm(dwg - dwo) <= 1e-2 * np.linalg.norm(dwo)
+
+
+@pytest.mark.gpu
+def test_gpu_learned_order_is_a_permutation_and_fallback(rc, wl):
+    print("[gpu] learned order", flush=True)
+    n, r, c, v = wl.grid_laplacian((10, 10), "loguniform", seed=2)
+    s = rc.Solver(device=0)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_LEARNED, 1)                 # no weights: nnz-sort fallback
+    t = rc.Solver(device=-1)
+    t.set_triplets(n, r, c, v)
+    t.order(rc.ORDER_NNZ_SORT, 1)
+    np.testing.assert_array_equal(s.permutation(), t.permutation())
+    w = go.init_weights(7)
+    s.set_gnn_weights(w)
+    s.order(rc.ORDER_LEARNED, 1)
+    p = s.permutation()
+    assert sorted(p.tolist()) == list(range(n))
+    sc = s.gnn_scores(w)
+    assert np.all(np.diff(sc[p]) <= 0)          # descending score
