@@ -105,3 +105,25 @@ def test_gpu_learned_order_is_a_permutation_and_fallback(rc, wl):
     assert sorted(p.tolist()) == list(range(n))
     sc = s.gnn_scores(w)
     assert np.all(np.diff(sc[p]) <= 0)          # descending score
+
+
+def test_plackett_luce_gradient_matches_finite_differences():
+    import importlib
+    from conftest import load_package
+    load_package()
+    tr = importlib.import_module("rchol_amd.gnn_train")
+    rng = np.random.default_rng(0)
+    s = rng.standard_normal(12)
+    perm = np.argsort(-(s + rng.gumbel(size=12)))
+
+    def logp(x):
+        xs = x[perm]
+        return float(sum(xs[i] - np.log(np.exp(xs[i:]).sum()) for i in range(len(xs))))
+
+    g = tr.pl_logp_grad(s, perm)
+    for i in range(12):
+        e = np.zeros(12); e[i] = 1e-6
+        fd = (logp(s + e) - logp(s - e)) / 2e-6
+        assert abs(fd - g[i]) < 1e-6
+    # the gradient of a log-probability sums to zero (shift invariance)
+    assert abs(g.sum()) < 1e-12
