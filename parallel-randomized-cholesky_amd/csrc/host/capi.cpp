@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <random>
@@ -103,6 +104,7 @@ struct rc_handle_s {
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
+    bool spatial_levels = false;
     DevVec tail_cptr;
     rc_factor_stats fstats{};
 
@@ -229,6 +231,10 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
     h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
     h->sweep_blocks = sweep_occupancy_blocks(o.device);
+    {
+        const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
+        h->spatial_levels = lo && std::string(lo) == "spatial";
+    }
     *out = h;
     return RC_OK;
 }
@@ -817,9 +823,20 @@ static int assemble(rc_handle_t h, double *ms_out)
     const int len_bits = lvl_bits + 16 <= 32 ? 16 : 0;
     RC_HIP(h->order.ensure(nsz * sizeof(int)));
     RC_HIP(h->pos_of.ensure(nsz * sizeof(int)));
-    launch_level_keys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, k0, s);
-    launch_iota(h->order.as<int>(), ns, s);
-    radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + len_bits, h->tmp_b.as<int>(), s);
+    if (h->spatial_levels) {
+        // rows of a level kept in the loaded (user) order: neighbouring rows of
+        // a chunk are neighbours in the input numbering, so gathers share lines
+        launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        launch_gather_i(h->level.as<int>(), h->order.as<int>(), (int *)k0, ns, s);
+        radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits, h->tmp_b.as<int>(), s);
+        launch_level_keys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, k1, s);
+        launch_gather_i((const int *)k1, h->order.as<int>(), (int *)k0, ns, s);   // sorted keys for the bounds
+    } else {
+        launch_level_keys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, k0, s);
+        launch_iota(h->order.as<int>(), ns, s);
+        radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + len_bits, h->tmp_b.as<int>(),
+                         s);
+    }
     launch_inverse_perm(ns, h->order.as<int>(), h->pos_of.as<int>(), s);
     const size_t nlsz = (size_t)nl + 1;
     int *lptr = h->tmp_a.as<int>();
