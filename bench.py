@@ -59,13 +59,17 @@ CONFIGS = {
                weights="loguniform", prec=32, rhs="normal_projected"),
     "c3_f64": dict(workload="grid3d_256_loguniform_nnzsort_f64", dims=(256, 256, 256), kind="laplacian",
                    weights="loguniform", prec=64, rhs="normal_projected"),
+    "c3_learned": dict(workload="grid3d_256_loguniform_learned_f32factor", dims=(256, 256, 256), kind="laplacian",
+                       weights="loguniform", prec=32, rhs="normal_projected", order="learned"),
+    "c4a": dict(workload="grid2d_2048_uniform_0.5_2_nnzsort_f32factor_64rhs", dims=(2048, 2048), kind="laplacian",
+                weights="uniform_0.5_2", prec=32, rhs="normal_projected", nrhs=64, wseed=11),
 }
 
 
 def make_system(wl, cfg):
     if cfg["kind"] == "sddm":
         return wl.grid_sddm(cfg["dims"], 1e-2)
-    return wl.grid_laplacian(cfg["dims"], cfg.get("weights", "unit"), seed=7)
+    return wl.grid_laplacian(cfg["dims"], cfg.get("weights", "unit"), seed=cfg.get("wseed", 7))
 
 
 def cpu_baseline(wl, cfg, b_full, tol, max_n):
@@ -136,17 +140,42 @@ def main():
     s = rc.Solver(device=0, seed=1, precision=cfg["prec"])
     t = time.perf_counter()
     s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if cfg["kind"] == "sddm" else rc.INPUT_LAPLACIAN)
-    s.order(rc.ORDER_NNZ_SORT, 1)
-    b = s.make_rhs(rc.RHS_NORMAL if cfg["rhs"] == "normal" else rc.RHS_NORMAL_PROJECTED, 42)
+    order_name = cfg.get("order", "nnz-sort")
+    if order_name == "learned":
+        sys.path.insert(0, PKG_DIR)
+        tr = importlib.import_module("rchol_amd.gnn_train")
+        w = tr.load_weights()
+        if w is None:
+            log("[bench] no trained weights: learned ordering falls back to nnz-sort")
+            order_name = "nnz-sort (learned fallback)"
+        else:
+            s.set_gnn_weights(w)
+        t_o = time.perf_counter()
+        s.order(rc.ORDER_LEARNED, 1)
+        log(f"[bench] learned ordering {time.perf_counter() - t_o:.2f}s")
+    else:
+        s.order(rc.ORDER_NNZ_SORT, 1)
+    nrhs = cfg.get("nrhs", 1)
+    rhs_kind = rc.RHS_NORMAL if cfg["rhs"] == "normal" else rc.RHS_NORMAL_PROJECTED
+    b = s.make_rhs(rhs_kind, 42)
+    B = np.stack([s.make_rhs(rhs_kind, 42 + q) for q in range(nrhs)], axis=1) if nrhs > 1 else None
     log(f"[bench] host input path (Laplacian, ordering, RHS) {time.perf_counter() - t:.1f}s")
 
     # warmup: first factor also prepares and uploads the system
     fst = sst = None
+    def solve_step():
+        if B is not None:
+            X, sts = s.solve_multi(B, tol=args.tol)
+            worst = max(sts, key=lambda d: d["iterations"])
+            return dict(worst, iterations=max(d["iterations"] for d in sts),
+                        rel_res_true=max(d["rel_res_true"] for d in sts))
+        return s.solve_resident(args.tol)
+
     for w in range(max(args.warmup, 1)):
         t = time.perf_counter()
         fst = s.factor()
         s.set_rhs(b)
-        sst = s.solve_resident(args.tol)
+        sst = solve_step()
         log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f}s iters={sst['iterations']} "
             f"factor={fst['factor_ms']:.1f}ms assemble={fst['assemble_ms']:.1f}ms pcg={sst['pcg_ms']:.1f}ms")
 
@@ -156,7 +185,7 @@ def main():
     for k in range(args.steps):
         t = time.perf_counter()
         fst = s.factor()
-        sst = s.solve_resident(args.tol)
+        sst = solve_step()
         times.append(time.perf_counter() - t)
         fms.append(fst["factor_ms"]); ams.append(fst["assemble_ms"]); pms.append(sst["pcg_ms"])
         iters.append(sst["iterations"])
@@ -193,7 +222,7 @@ def main():
         "dtype": "f64" if cfg["prec"] == 64 else "f32-factor/f64-pcg",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "n": int(s.size()[0]), "nnz": int(s.size()[1]),
-                   "factor_precision": cfg["prec"], "order": "nnz-sort", "tol": args.tol,
+                   "factor_precision": cfg["prec"], "order": order_name, "nrhs": nrhs, "tol": args.tol,
                    "stop_rule": "reference sqrt(r'z/r0'z0)"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None},
