@@ -63,13 +63,59 @@ CONFIGS = {
                        weights="loguniform", prec=32, rhs="normal_projected", order="learned"),
     "c4a": dict(workload="grid2d_2048_uniform_0.5_2_nnzsort_f32factor_64rhs", dims=(2048, 2048), kind="laplacian",
                 weights="uniform_0.5_2", prec=32, rhs="normal_projected", nrhs=64, wseed=11),
+    "c4b": dict(workload="rgg_batch_200_n10k-200k_nnzsort_f32factor", kind="rgg_batch", nsys=200, prec=32,
+                rhs="normal_projected"),
 }
+
+
+def rgg_batch_sizes(nsys):
+    """n_i = 10,000 + floor(i * 190,000 / 199) for i < 200."""
+    return [10_000 + (i * 190_000) // max(nsys - 1, 1) for i in range(nsys)]
+
+
+def make_batch(wl, cfg, count=None):
+    sizes = rgg_batch_sizes(cfg["nsys"])[:count]
+    return [wl.random_geometric_graph(m, seed=1000 + i) for i, m in enumerate(sizes)]
 
 
 def make_system(wl, cfg):
     if cfg["kind"] == "sddm":
         return wl.grid_sddm(cfg["dims"], 1e-2)
     return wl.grid_laplacian(cfg["dims"], cfg.get("weights", "unit"), seed=cfg.get("wseed", 7))
+
+
+def cpu_baseline_batch(wl, cfg, tol, max_n):
+    """The oracle on the first systems of the batch, each solved alone, until
+    max_n rows are covered; time scaled by total rows / sampled rows."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    sizes = rgg_batch_sizes(cfg["nsys"])
+    total = sum(sizes)
+    t_f = t_p = 0.0
+    rows = 0
+    its = []
+    i = 0
+    while i < len(sizes) and rows < max_n:
+        n, r, c, v = wl.random_geometric_graph(sizes[i], seed=1000 + i)
+        fac, sys_, perm, _ = orc.prepare_system(n, r, c, v, kind="laplacian", order="nnz-sort", order_seed=1)
+        b = np.array([orc.normal(42, j) for j in range(n)])
+        b -= b.mean()
+        t0 = time.perf_counter()
+        f = orc.factor(fac, seed=1)
+        t1 = time.perf_counter()
+        _, st = orc.pcg(sys_, f, b[perm], tol=tol, precision=cfg["prec"])
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_p += t2 - t1
+        rows += n
+        its.append(st["iterations"])
+        i += 1
+    scale = total / rows
+    sample = (f"oracle (1 thread) factor+PCG of the first {i} systems ({rows} rows) one by one: factor {t_f:.2f}s "
+              f"+ PCG {t_p:.2f}s, iterations {min(its)}-{max(its)}; scaled x{scale:.1f} to {total} rows")
+    return {"value": (t_f + t_p) * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
+            "factor_s": t_f, "pcg_s": t_p}
 
 
 def cpu_baseline(wl, cfg, b_full, tol, max_n):
@@ -134,12 +180,21 @@ def main():
         cfg["prec"] = args.precision
     log(f"[bench] config {args.config}: generating {cfg['workload']}")
     t = time.perf_counter()
-    n, r, c, v = make_system(wl, cfg)
-    log(f"[bench] generated n={n} triplets={len(r)} in {time.perf_counter() - t:.1f}s")
+    batch = cfg["kind"] == "rgg_batch"
+    if batch:
+        systems = make_batch(wl, cfg)
+        log(f"[bench] generated {len(systems)} systems, n={sum(q[0] for q in systems)} in {time.perf_counter() - t:.1f}s")
+    else:
+        n, r, c, v = make_system(wl, cfg)
+        log(f"[bench] generated n={n} triplets={len(r)} in {time.perf_counter() - t:.1f}s")
 
     s = rc.Solver(device=0, seed=1, precision=cfg["prec"])
     t = time.perf_counter()
-    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if cfg["kind"] == "sddm" else rc.INPUT_LAPLACIAN)
+    if batch:
+        s.set_batch(systems)
+        del systems
+    else:
+        s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if cfg["kind"] == "sddm" else rc.INPUT_LAPLACIAN)
     order_name = cfg.get("order", "nnz-sort")
     if order_name == "learned":
         sys.path.insert(0, PKG_DIR)
@@ -222,7 +277,8 @@ def main():
         "dtype": "f64" if cfg["prec"] == 64 else "f32-factor/f64-pcg",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "n": int(s.size()[0]), "nnz": int(s.size()[1]),
-                   "factor_precision": cfg["prec"], "order": order_name, "nrhs": nrhs, "tol": args.tol,
+                   "factor_precision": cfg["prec"], "order": order_name, "nrhs": nrhs,
+                   "systems": s.batch()[0], "tol": args.tol,
                    "stop_rule": "reference sqrt(r'z/r0'z0)"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None},
@@ -236,7 +292,8 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] CPU oracle baseline ...")
-        res["cpu_baseline"] = cpu_baseline(wl, cfg, b, args.tol, args.cpu_max_n)
+        res["cpu_baseline"] = (cpu_baseline_batch(wl, cfg, args.tol, args.cpu_max_n) if batch
+                               else cpu_baseline(wl, cfg, b, args.tol, args.cpu_max_n))
         log(f"[bench] cpu: {res['cpu_baseline']['sample']}")
     else:
         res["cpu_baseline"] = None

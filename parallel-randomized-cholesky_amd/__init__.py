@@ -39,6 +39,7 @@ EXPORTS = (
     "rc_solve", "rc_make_rhs", "rc_set_reference_solution", "rc_set_rhs", "rc_solve_resident",
     "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
     "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
+    "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats",
 )
 
 
@@ -130,6 +131,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_gnn_weight_count.restype = ctypes.c_int64
     lib.rc_gnn_scores.argtypes = [vp, vp, vp]
     lib.rc_gnn_gradient.argtypes = [vp, vp, vp, vp]
+    lib.rc_set_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
+    lib.rc_get_batch.argtypes = [vp, P(ctypes.c_int), vp]
+    lib.rc_solve_batch.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    lib.rc_get_batch_stats.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -188,6 +193,40 @@ class Solver:
         c = np.ascontiguousarray(cols, dtype=np.int32)
         v = np.ascontiguousarray(vals, dtype=np.float64)
         self._check(self.lib.rc_set_triplets(self.h, int(n), len(r), _ptr(r), _ptr(c), _ptr(v), kind))
+
+    def set_batch(self, systems):
+        """systems: list of (n, rows, cols, vals) with system-local indices."""
+        ns = np.array([int(t[0]) for t in systems], dtype=np.int32)
+        ms = np.array([len(t[1]) for t in systems], dtype=np.int64)
+        r = np.ascontiguousarray(np.concatenate([np.asarray(t[1]) for t in systems]), dtype=np.int32)
+        c = np.ascontiguousarray(np.concatenate([np.asarray(t[2]) for t in systems]), dtype=np.int32)
+        v = np.ascontiguousarray(np.concatenate([np.asarray(t[3]) for t in systems]), dtype=np.float64)
+        self._check(self.lib.rc_set_batch(self.h, len(systems), _ptr(ns), _ptr(ms), _ptr(r), _ptr(c), _ptr(v)))
+
+    def batch(self):
+        """(number of systems, offsets[nsys + 1]) of the loaded input."""
+        k = ctypes.c_int()
+        self._check(self.lib.rc_get_batch(self.h, ctypes.byref(k), None))
+        off = np.zeros(k.value + 1, dtype=np.int64)
+        self._check(self.lib.rc_get_batch(self.h, ctypes.byref(k), _ptr(off)))
+        return k.value, off
+
+    def solve_batch(self, b, tol=1e-6, allow_breakdown=False):
+        """One PCG per system; returns x and the list of per-system stats."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(b)
+        k, _ = self.batch()
+        st = (SolveStats * k)()
+        rc = self.lib.rc_solve_batch(self.h, _ptr(b), _ptr(x), float(tol), st)
+        if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
+            self._check(rc)
+        return x, [st[q].as_dict() for q in range(k)]
+
+    def batch_stats(self):
+        k, _ = self.batch()
+        st = (SolveStats * k)()
+        self._check(self.lib.rc_get_batch_stats(self.h, st))
+        return [st[q].as_dict() for q in range(k)]
 
     def size(self):
         n, nnz = ctypes.c_int64(), ctypes.c_int64()

@@ -174,6 +174,20 @@ __global__ void k_level_keys(int ns, const int *level, const int *lptr, int len_
     }
 }
 
+// Batched systems: (level, system, descending row length) so that the rows
+// of one system inside a level are contiguous.
+__global__ void k_level_keys_sys(int ns, const int *level, const int *lptr, int len_bits, const int *sys, int sys_bits,
+                                 uint32_t *keys)
+{
+    const int lmax = (1 << len_bits) - 1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x) {
+        int len = lptr[k + 1] - lptr[k];
+        if (len > lmax) len = lmax;
+        uint32_t ls = ((uint32_t)level[k] << sys_bits) | (uint32_t)sys[k];
+        keys[k] = (ls << len_bits) | (uint32_t)(lmax - len);
+    }
+}
+
 __global__ void k_inverse_perm(int n, const int *order, int *pos_of)
 {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) pos_of[order[p]] = p;
@@ -372,6 +386,11 @@ void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int
 void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s)
 {
     hipLaunchKernelGGL(k_level_keys, dim3(2048), dim3(256), 0, s, ns, level, lptr, len_bits, keys);
+}
+void launch_level_keys_sys(int ns, const int *level, const int *lptr, int len_bits, const int *sys, int sys_bits,
+                           uint32_t *keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_keys_sys, dim3(2048), dim3(256), 0, s, ns, level, lptr, len_bits, sys, sys_bits, keys);
 }
 void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s)
 {

@@ -190,6 +190,10 @@ struct SolveArgs {
     const int *tail_cptr;
     long long spin_ticks;
     const double *b;
+    // batched systems: system of every position and the per-system alpha
+    // used by the forward sweep's r/x update (both null for one system)
+    const int *sys_of;
+    const double *alpha_sys;
 };
 int sweep_occupancy_blocks(int device);
 
@@ -264,6 +268,36 @@ template <typename T> void launch_multi_iteration(const MultiArgs &a, int parity
 void launch_multi_residual(const MultiArgs &a, hipStream_t s);
 void launch_multi_gather(const double *B, const int *idx_user, int n, int k, int K, double *b, hipStream_t s);
 void launch_multi_scatter(const double *x, const int *idx_user, int n, int k, int K, double *X, hipStream_t s);
+
+// ------------------------------------------------------------ batched systems
+// nsys independent Laplacians stored as one block-diagonal system; every
+// system runs its own PCG (own alpha, beta, r'z, stop test, iteration count).
+// Reductions are segmented: positions of system s are listed (ascending) in
+// spos[q], q in [qoff[s], qoff[s+1]); reduction block b covers
+// q in [blk_lo[b], blk_hi[b]) of system blk_sys[b]; the blocks of system s are
+// sys_blk0[s] .. sys_blk0[s+1]-1, and the last of them to finish reduces the
+// block partials in block order (deterministic) and updates the scalars.
+constexpr int kBatchRows = 2048;          // positions per reduction block
+struct BatchArgs {
+    int nsys, nblk;
+    const int *sys_of;                    // position -> system
+    const int *spos;                      // system-major positions
+    const int *blk_lo, *blk_hi, *blk_sys, *sys_blk0;
+    double *rz, *rz0, *alpha, *beta, *bb, *rr, *res2, *bn2;
+    int *iter, *done, *brk;
+    unsigned int *arrive;                 // per system
+    unsigned int *arrive_all;             // systems finalised this pass
+    double *part;                         // 2 * nblk
+};
+// Device bytes of the per-system state of nsys systems and nblk blocks.
+size_t batch_state_bytes(int nsys, int nblk);
+// Carve BatchArgs pointers out of one buffer of batch_state_bytes bytes.
+void batch_carve(BatchArgs &b, void *buf);
+template <typename T> void launch_batch_init(const SolveArgs &a, const BatchArgs &b, hipStream_t s);
+template <typename T> void launch_batch_iteration(const SolveArgs &a, const BatchArgs &b, int parity, hipStream_t s);
+void launch_batch_residual(const SolveArgs &a, const BatchArgs &b, hipStream_t s);
+void launch_level_keys_sys(int ns, const int *level, const int *lptr, int len_bits, const int *sys, int sys_bits,
+                           uint32_t *keys, hipStream_t s);
 
 // CSR in position numbering: row p = source row order[p], columns mapped by
 // src.colmap, values converted to TD.

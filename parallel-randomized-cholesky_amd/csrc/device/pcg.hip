@@ -236,6 +236,7 @@ __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int c, int t, doub
     if (active && g == 0) {
         double rk = a.r[k];
         if (update_rx) {
+            if (a.alpha_sys) alpha = a.alpha_sys[a.sys_of[k]];
             rk = fma_t(-alpha, a.ap[k], rk);
             a.r[k] = rk;
             a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
@@ -497,6 +498,181 @@ void launch_bwd(const SolveArgs &a, hipStream_t s)
                            (const double *)nullptr, 0);
 }
 
+// ---------------------------------------------------------------- batched systems
+// Block-diagonal system of nsys independent Laplacians (no entry couples two
+// systems), one PCG per system.  The SpMV and the sweeps run over the whole
+// batch in one launch each; alpha and beta are looked up per row through
+// sys_of, and the per-system dot products are segmented reductions over the
+// system-major position list (see BatchArgs).  A converged system is frozen:
+// its alpha and beta are zero, so x and r keep their values and p stays z.
+enum { kOpPap = 0, kOpRz = 1, kOpBnorm = 2, kOpRes = 3 };
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, BatchArgs b, const double *p_old, double *p_new,
+                                                     int first)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    const T *z = (const T *)a.z;
+    const int lane = lane_id();
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
+        const int tile = p / kWave;
+        const int w = a.a_width[tile];
+        const int *ind = a.a_ind + a.a_off[tile] + lane;
+        const double *val = a.a_val + a.a_off[tile] + lane;
+        // every neighbour of row p belongs to p's system, so p's beta applies
+        const double beta = (first || p >= a.n) ? 0.0 : b.beta[b.sys_of[p]];
+        double acc = 0.0;
+        int t = 0;
+        for (; t + 4 <= w; t += 4) {
+            int j[4];
+            double v[4], pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { j[u] = ind[(t + u) * kWave]; v[u] = val[(t + u) * kWave]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = beta * p_old[j[u]] + (double)z[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
+        }
+        for (; t < w; ++t) {
+            int j = ind[t * kWave];
+            acc = fma_t(val[t * kWave], beta * p_old[j] + (double)z[j], acc);
+        }
+        if (p < a.n) {
+            p_new[p] = beta * p_old[p] + (double)z[p];
+            a.ap[p] = acc;
+        }
+    }
+}
+
+// ap = A x - b (for the final residuals).
+__global__ void __launch_bounds__(kRedBlock) k_bresid(SolveArgs a)
+{
+    const int lane = lane_id();
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
+        const int tile = p / kWave;
+        const int w = a.a_width[tile];
+        const int *ind = a.a_ind + a.a_off[tile] + lane;
+        const double *val = a.a_val + a.a_off[tile] + lane;
+        double acc = 0.0;
+        for (int t = 0; t < w; ++t) acc = fma_t(val[t * kWave], a.x[ind[t * kWave]], acc);
+        if (p < a.n) a.ap[p] = acc - a.b[p];
+    }
+}
+
+// One block per reduction block of the table (grid = nblk).  Op selects the
+// sums (pap: p'Ap; rz: r'z and r'r; bnorm: b'b; res: d'd and b'b with d in ap)
+// and the per-system update made by the system's last block.
+template <typename T, int Op>
+__global__ void __launch_bounds__(kRedBlock) k_bseg(SolveArgs a, BatchArgs b, const double *pv, int init)
+{
+    __shared__ double lds[kRedBlock / kWave];
+    __shared__ int flag;
+    PcgScalars *sc = a.sc;
+    if ((Op == kOpPap || Op == kOpRz) && sc->done) return;
+    const int blk = blockIdx.x;
+    const int s = b.blk_sys[blk];
+    const bool need_rr = sc->stop_rule == 1;
+    const bool skip = (Op == kOpPap || (Op == kOpRz && !init)) && b.done[s] != 0;
+    double acc = 0.0, acc2 = 0.0;
+    if (!skip) {
+        const T *z = (const T *)a.z;
+        for (int q = b.blk_lo[blk] + threadIdx.x; q < b.blk_hi[blk]; q += blockDim.x) {
+            const int p = b.spos[q];
+            if (Op == kOpPap) {
+                acc += pv[p] * a.ap[p];
+            } else if (Op == kOpRz) {
+                const double ri = a.r[p];
+                acc += ri * (double)z[p];
+                if (need_rr) acc2 += ri * ri;
+            } else if (Op == kOpBnorm) {
+                const double bi = a.b[p];
+                acc += bi * bi;
+            } else {
+                const double d = a.ap[p], bi = a.b[p];
+                acc += d * d;
+                acc2 += bi * bi;
+            }
+        }
+    }
+    const double s1 = block_sum(acc, lds);
+    const double s2 = block_sum(acc2, lds);
+    if (threadIdx.x == 0) {
+        st_relaxed_val(b.part + blk, s1);
+        st_relaxed_val(b.part + b.nblk + blk, s2);
+    }
+    __syncthreads();
+    const int b0 = b.sys_blk0[s], b1 = b.sys_blk0[s + 1];
+    if (threadIdx.x == 0) {
+        fence_release();
+        const unsigned int old = atomicAdd(b.arrive + s, 1u);
+        flag = (old == (unsigned int)(b1 - b0 - 1)) ? 1 : 0;
+        if (flag) fence_acquire();
+    }
+    __syncthreads();
+    if (!flag) return;
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        t1 += ld_relaxed_val(b.part + i);
+        t2 += ld_relaxed_val(b.part + b.nblk + i);
+    }
+    t1 = block_sum(t1, lds);
+    t2 = block_sum(t2, lds);
+    if (threadIdx.x != 0) return;
+    b.arrive[s] = 0u;
+    const double tol = sc->tol;
+    if (Op == kOpPap) {
+        if (b.done[s]) {
+            b.alpha[s] = 0.0;
+        } else if (!(t1 > 0.0) || !isfinite(t1)) {        // breakdown of this system
+            b.brk[s] = 1;
+            b.done[s] = 1;
+            b.alpha[s] = 0.0;
+            b.beta[s] = 0.0;
+        } else {
+            b.alpha[s] = b.rz[s] / t1;
+        }
+        return;
+    }
+    if (Op == kOpBnorm) { b.bb[s] = t1; return; }
+    if (Op == kOpRes) { b.res2[s] = t1; b.bn2[s] = t2; return; }
+    // kOpRz: per-system beta and stop test (solver.hpp:493, 536), then the
+    // last system to finish decides whether the batch is done
+    if (init) {
+        b.rz0[s] = t1;
+        b.rz[s] = t1;
+        b.rr[s] = t2;
+        b.beta[s] = 0.0;
+        b.alpha[s] = 0.0;
+        b.iter[s] = 0;
+        const bool go = need_rr ? (b.bb[s] > 0.0 && sqrt(t2) / sqrt(b.bb[s]) > tol) : (t1 > tol);
+        b.done[s] = go ? 0 : 1;
+    } else if (!b.done[s]) {
+        b.beta[s] = t1 / b.rz[s];
+        b.rz[s] = t1;
+        b.rr[s] = t2;
+        b.iter[s] += 1;
+        const bool go = need_rr ? (sqrt(t2) / sqrt(b.bb[s]) > tol) : (sqrt(t1) / sqrt(b.rz0[s]) > tol);
+        if (!go || b.iter[s] >= sc->maxit) {
+            b.done[s] = 1;
+            b.beta[s] = 0.0;
+        }
+    }
+    fence_release();
+    const unsigned int old = atomicAdd(b.arrive_all, 1u);
+    if (old != (unsigned int)(b.nsys - 1)) return;
+    fence_acquire();
+    *b.arrive_all = 0u;
+    int all = 1;
+    for (int q = 0; q < b.nsys; ++q) all &= (ld_relaxed(b.done + q) != 0) ? 1 : 0;
+    if (init) sc->iter = 0;
+    else sc->iter += 1;
+    if (all || sc->iter >= sc->maxit || sc->status != 0) sc->done = 1;
+}
+
+int batch_grid(const BatchArgs &b) { return b.nblk > 0 ? b.nblk : 1; }
+
 }  // namespace
 
 template <typename T>
@@ -577,6 +753,66 @@ void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s)
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, a, b);
 }
 
+size_t batch_state_bytes(int nsys, int nblk)
+{
+    // 8 double arrays + 3 int arrays + arrive (nsys) + arrive_all, then 2 nblk partials
+    return (size_t)nsys * (8 * sizeof(double) + 4 * sizeof(int)) + 64 + (size_t)2 * nblk * sizeof(double) + 256;
+}
+
+void batch_carve(BatchArgs &b, void *buf)
+{
+    char *c = (char *)buf;
+    const size_t S = (size_t)b.nsys;
+    double **d[8] = {&b.rz, &b.rz0, &b.alpha, &b.beta, &b.bb, &b.rr, &b.res2, &b.bn2};
+    for (int i = 0; i < 8; ++i) { *d[i] = (double *)c; c += S * sizeof(double); }
+    b.part = (double *)c;
+    c += (size_t)2 * b.nblk * sizeof(double);
+    b.iter = (int *)c; c += S * sizeof(int);
+    b.done = (int *)c; c += S * sizeof(int);
+    b.brk = (int *)c; c += S * sizeof(int);
+    b.arrive = (unsigned int *)c; c += S * sizeof(unsigned int);
+    b.arrive_all = (unsigned int *)c;
+}
+
+template <typename T>
+void launch_batch_init(const SolveArgs &a, const BatchArgs &b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.y, a.n);
+    hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, a.n);
+    hipLaunchKernelGGL(k_pcg_prepare<T>, dim3(2048), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_bseg<T, kOpBnorm>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b,
+                       (const double *)nullptr, 1);
+    launch_fwd<T>(a, (const double *)a.p0, 0, s);
+    launch_bwd<T>(a, s);
+    hipLaunchKernelGGL((k_bseg<T, kOpRz>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b, (const double *)nullptr,
+                       1);
+}
+
+template <typename T>
+void launch_batch_iteration(const SolveArgs &a, const BatchArgs &b, int parity, hipStream_t s)
+{
+    double *p_old = parity ? a.p1 : a.p0;
+    double *p_new = parity ? a.p0 : a.p1;
+    hipLaunchKernelGGL(k_bspmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, b, (const double *)p_old, p_new, 0);
+    hipLaunchKernelGGL((k_bseg<T, kOpPap>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b, (const double *)p_new,
+                       0);
+    launch_fwd<T>(a, (const double *)p_new, 1, s);
+    launch_bwd<T>(a, s);
+    hipLaunchKernelGGL((k_bseg<T, kOpRz>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b, (const double *)nullptr,
+                       0);
+}
+
+void launch_batch_residual(const SolveArgs &a, const BatchArgs &b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bresid, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a);
+    hipLaunchKernelGGL((k_bseg<double, kOpRes>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b,
+                       (const double *)nullptr, 0);
+}
+
+template void launch_batch_init<float>(const SolveArgs &, const BatchArgs &, hipStream_t);
+template void launch_batch_init<double>(const SolveArgs &, const BatchArgs &, hipStream_t);
+template void launch_batch_iteration<float>(const SolveArgs &, const BatchArgs &, int, hipStream_t);
+template void launch_batch_iteration<double>(const SolveArgs &, const BatchArgs &, int, hipStream_t);
 template void launch_fill_sentinel<float>(float *, int, hipStream_t);
 template void launch_fill_sentinel<double>(double *, int, hipStream_t);
 template void launch_pcg_init<float>(const SolveArgs &, hipStream_t);

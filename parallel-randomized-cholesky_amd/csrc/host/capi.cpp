@@ -123,6 +123,14 @@ struct rc_handle_s {
     int gnn_n = 0;
     DevVec g_ptr, g_ind, g_a, g_at, g_H0, g_M1, g_H[3], g_M[3], g_Z[3], g_dA, g_dB, g_dC, g_part, g_w, g_dw, g_s, g_ds;
 
+    // batched systems (nsys == 0: one system)
+    int nsys = 0;
+    std::vector<int64_t> sys_off;   // nsys + 1, user numbering
+    std::vector<int> sys_user;      // user row -> system
+    int nblk = 0;
+    DevVec sys_user_d, sys_fac, sys_pos, spos, blk_tab, bt_state;
+    std::vector<rc_solve_stats> bstats;
+
     // multi-RHS: CSRs in position numbering (built lazily per factor) + vectors
     bool multi_ready = false;
     DevVec m_aptr, m_aind, m_aval, m_lptr, m_lind, m_lval, m_ltptr, m_ltind, m_ltval;
@@ -264,6 +272,10 @@ static void reset_system(rc_handle_t h)
     h->rhs_set = false;
     h->perm.clear();
     h->xref.clear();
+    h->nsys = 0;
+    h->sys_off.clear();
+    h->sys_user.clear();
+    h->bstats.clear();
     invalidate_graph(h);
 }
 
@@ -282,6 +294,55 @@ int rc_set_triplets(rc_handle_t h, int n, int64_t m, const int *rows, const int 
     h->base = std::move(a);
     h->kind = kind;
     h->n_user = kind == RC_INPUT_GROUNDED ? n - 1 : n;
+    return RC_OK;
+}
+
+int rc_set_batch(rc_handle_t h, int nsys, const int *sys_n, const int64_t *sys_m, const int *rows, const int *cols,
+                 const double *vals)
+{
+    RC_CHECK_HANDLE();
+    if (nsys <= 0 || !sys_n || !sys_m) return fail(h, RC_ERR_ARG, "bad batch description");
+    int64_t n = 0, m = 0;
+    for (int s = 0; s < nsys; ++s) {
+        if (sys_n[s] <= 0 || sys_m[s] < 0) return fail(h, RC_ERR_ARG, "every system needs n >= 1 and m >= 0");
+        n += sys_n[s];
+        m += sys_m[s];
+    }
+    if (n > 2147483646LL) return fail(h, RC_ERR_ARG, "batch exceeds int32 rows");
+    if (m > 0 && (!rows || !cols || !vals)) return fail(h, RC_ERR_ARG, "bad triplet arrays");
+    // system-local indices -> block-diagonal numbering
+    std::vector<int> r((size_t)m), c((size_t)m);
+    int64_t q = 0, off = 0;
+    for (int s = 0; s < nsys; ++s) {
+        for (int64_t e = 0; e < sys_m[s]; ++e, ++q) {
+            if (rows[q] < 0 || rows[q] >= sys_n[s] || cols[q] < 0 || cols[q] >= sys_n[s])
+                return fail(h, RC_ERR_ARG, "triplet index outside its system (system " + std::to_string(s) + ")");
+            r[(size_t)q] = (int)(rows[q] + off);
+            c[(size_t)q] = (int)(cols[q] + off);
+        }
+        off += sys_n[s];
+    }
+    int rc = rc_set_triplets(h, (int)n, m, r.data(), c.data(), vals, RC_INPUT_LAPLACIAN);
+    if (rc != RC_OK) return rc;
+    h->nsys = nsys;
+    h->sys_off.assign((size_t)nsys + 1, 0);
+    h->sys_user.resize((size_t)n);
+    for (int s = 0; s < nsys; ++s) {
+        h->sys_off[(size_t)s + 1] = h->sys_off[(size_t)s] + sys_n[s];
+        for (int64_t i = h->sys_off[(size_t)s]; i < h->sys_off[(size_t)s + 1]; ++i) h->sys_user[(size_t)i] = s;
+    }
+    return RC_OK;
+}
+
+int rc_get_batch(rc_handle_t h, int *nsys, int64_t *offsets)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (nsys) *nsys = h->nsys > 0 ? h->nsys : 1;
+    if (offsets) {
+        if (h->nsys > 0) std::memcpy(offsets, h->sys_off.data(), h->sys_off.size() * sizeof(int64_t));
+        else { offsets[0] = 0; offsets[1] = h->n_user; }
+    }
     return RC_OK;
 }
 
@@ -575,6 +636,39 @@ static int prepare(rc_handle_t h)
     }
     RC_HIP(h->perm_d.ensure((size_t)std::max(n, 1) * sizeof(int)));
     RC_HIP(hipMemcpyAsync(h->perm_d.p, h->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (h->nsys > 0) {
+        // system of every row (user and factored numbering) and the
+        // reduction-block table: system s owns q in [sys_off[s], sys_off[s+1])
+        // of the system-major position list
+        std::vector<int> sys_fac((size_t)n);
+        for (int i = 0; i < n; ++i) sys_fac[(size_t)i] = h->sys_user[(size_t)h->perm[(size_t)i]];
+        std::vector<int> lo, hi, bs, b0((size_t)h->nsys + 1);
+        for (int q = 0; q < h->nsys; ++q) {
+            b0[(size_t)q] = (int)lo.size();
+            for (int64_t a0 = h->sys_off[(size_t)q]; a0 < h->sys_off[(size_t)q + 1]; a0 += kBatchRows) {
+                lo.push_back((int)a0);
+                hi.push_back((int)std::min<int64_t>(a0 + kBatchRows, h->sys_off[(size_t)q + 1]));
+                bs.push_back(q);
+            }
+        }
+        b0[(size_t)h->nsys] = (int)lo.size();
+        h->nblk = (int)lo.size();
+        std::vector<int> tab;
+        tab.reserve(3 * lo.size() + b0.size());
+        tab.insert(tab.end(), lo.begin(), lo.end());
+        tab.insert(tab.end(), hi.begin(), hi.end());
+        tab.insert(tab.end(), bs.begin(), bs.end());
+        tab.insert(tab.end(), b0.begin(), b0.end());
+        RC_HIP(h->sys_user_d.ensure((size_t)n * sizeof(int)));
+        RC_HIP(h->sys_fac.ensure((size_t)n * sizeof(int)));
+        RC_HIP(h->blk_tab.ensure(tab.size() * sizeof(int)));
+        RC_HIP(h->bt_state.ensure(batch_state_bytes(h->nsys, h->nblk)));
+        RC_HIP(hipMemcpyAsync(h->sys_user_d.p, h->sys_user.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                              h->stream));
+        RC_HIP(hipMemcpyAsync(h->sys_fac.p, sys_fac.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        RC_HIP(hipMemcpyAsync(h->blk_tab.p, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        RC_HIP(hipStreamSynchronize(h->stream));
+    }
     RC_HIP(hipStreamSynchronize(h->stream));
     h->prepared = true;
     h->factored = false;
@@ -820,10 +914,19 @@ static int assemble(rc_handle_t h, double *ms_out)
     const int nl = ns > 0 ? (int)maxlev + 1 : 0;
     h->nlevels = nl;
     const int lvl_bits = bit_width(nl);
-    const int len_bits = lvl_bits + 16 <= 32 ? 16 : 0;
+    // batched systems: (level, system, length) keys keep a system's rows of a
+    // level contiguous, so the segmented reductions read runs of positions
+    const int sys_bits = (h->nsys > 1 && lvl_bits + bit_width(h->nsys - 1) <= 32) ? bit_width(h->nsys - 1) : 0;
+    const int len_bits = std::max(0, std::min(16, 32 - lvl_bits - sys_bits));
     RC_HIP(h->order.ensure(nsz * sizeof(int)));
     RC_HIP(h->pos_of.ensure(nsz * sizeof(int)));
-    if (h->spatial_levels) {
+    if (h->nsys > 0) {
+        launch_level_keys_sys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, h->sys_fac.as<int>(), sys_bits, k0,
+                              s);
+        launch_iota(h->order.as<int>(), ns, s);
+        radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + sys_bits + len_bits,
+                         h->tmp_b.as<int>(), s);
+    } else if (h->spatial_levels) {
         // rows of a level kept in the loaded (user) order: neighbouring rows of
         // a chunk are neighbours in the input numbering, so gathers share lines
         launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
@@ -842,7 +945,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     int *lptr = h->tmp_a.as<int>();
     int *lcnt = lptr + nlsz;
     int *cptr = lcnt + nlsz;
-    launch_segment_bounds(ns, k0, len_bits, nl, lptr, s);
+    launch_segment_bounds(ns, k0, len_bits + sys_bits, nl, lptr, s);
     launch_chunk_count(nl, lptr, lcnt, s);
     scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
     int nchunks = 0;
@@ -931,6 +1034,17 @@ static int assemble(rc_handle_t h, double *ms_out)
     // user index of every position (RHS gather / solution scatter)
     RC_HIP(h->idx_user.ensure(nsz * sizeof(int)));
     launch_gather_i(h->perm_d.as<int>(), h->order.as<int>(), h->idx_user.as<int>(), ns, s);
+    if (h->nsys > 0) {
+        // system of every position, and the positions listed system by system
+        RC_HIP(h->sys_pos.ensure(nsz * sizeof(int)));
+        RC_HIP(h->spos.ensure(nsz * sizeof(int)));
+        launch_gather_i(h->sys_user_d.as<int>(), h->idx_user.as<int>(), h->sys_pos.as<int>(), ns, s);
+        launch_copy_keys(h->sys_pos.as<int>(), k0, ns, s);
+        launch_iota(h->vals0.as<int>(), ns, s);
+        radix_sort_pairs(k0, h->vals0.as<int>(), k1, h->vals1.as<int>(), ns, std::max(1, bit_width(h->nsys - 1)),
+                         h->tmp_b.as<int>(), s);
+        RC_HIP(hipMemcpyAsync(h->spos.p, h->vals0.p, (size_t)ns * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
 
     RC_HIP(hipGetLastError());
     RC_HIP(hipEventRecord(e1, s));
@@ -1222,7 +1336,33 @@ static SolveArgs solve_args(rc_handle_t h)
     a.tail_cptr = h->tail_cptr.as<int>();
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
+    a.sys_of = nullptr;
+    a.alpha_sys = nullptr;
+    if (h->nsys > 0) {
+        BatchArgs b;
+        b.nsys = h->nsys;
+        b.nblk = h->nblk;
+        batch_carve(b, h->bt_state.p);
+        a.sys_of = h->sys_pos.as<int>();
+        a.alpha_sys = b.alpha;
+    }
     return a;
+}
+
+static BatchArgs batch_args(rc_handle_t h)
+{
+    BatchArgs b;
+    b.nsys = h->nsys;
+    b.nblk = h->nblk;
+    batch_carve(b, h->bt_state.p);
+    b.sys_of = h->sys_pos.as<int>();
+    b.spos = h->spos.as<int>();
+    const int *tab = h->blk_tab.as<int>();
+    b.blk_lo = tab;
+    b.blk_hi = tab + h->nblk;
+    b.blk_sys = tab + 2 * h->nblk;
+    b.sys_blk0 = tab + 3 * h->nblk;
+    return b;
 }
 
 static int alloc_solve(rc_handle_t h)
@@ -1295,7 +1435,16 @@ static int build_graph(rc_handle_t h)
     SolveArgs a = solve_args(h);
     hipGraph_t g;
     RC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    if (h->prec == 32) {
+    if (h->nsys > 0) {
+        BatchArgs b = batch_args(h);
+        if (h->prec == 32) {
+            launch_batch_iteration<float>(a, b, 0, h->stream);
+            launch_batch_iteration<float>(a, b, 1, h->stream);
+        } else {
+            launch_batch_iteration<double>(a, b, 0, h->stream);
+            launch_batch_iteration<double>(a, b, 1, h->stream);
+        }
+    } else if (h->prec == 32) {
         launch_pcg_iteration<float>(a, 0, h->stream);
         launch_pcg_iteration<float>(a, 1, h->stream);
     } else {
@@ -1343,8 +1492,16 @@ static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
     SolveArgs a = solve_args(h);
     RC_HIP(hipEventRecord(e0, s));
     RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
-    if (h->prec == 32) launch_pcg_init<float>(a, s);
-    else launch_pcg_init<double>(a, s);
+    if (h->nsys > 0) {
+        BatchArgs b = batch_args(h);
+        RC_HIP(hipMemsetAsync(h->bt_state.p, 0, batch_state_bytes(h->nsys, h->nblk), s));
+        if (h->prec == 32) launch_batch_init<float>(a, b, s);
+        else launch_batch_init<double>(a, b, s);
+    } else if (h->prec == 32) {
+        launch_pcg_init<float>(a, s);
+    } else {
+        launch_pcg_init<double>(a, s);
+    }
     RC_HIP(hipGetLastError());
     RC_HIP(hipEventRecord(e1, s));
     // iterations: 2 per graph launch; poll the done flag every 4 launches
@@ -1367,8 +1524,14 @@ static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
     }
     (void)hipHostFree(done_h);
     RC_HIP(hipEventRecord(e2, s));
-    launch_residual(a, s);
+    if (h->nsys > 0) launch_batch_residual(a, batch_args(h), s);
+    else launch_residual(a, s);
     RC_HIP(hipMemcpyAsync(&sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+    std::vector<char> bt;
+    if (h->nsys > 0) {
+        bt.resize(batch_state_bytes(h->nsys, h->nblk));
+        RC_HIP(hipMemcpyAsync(bt.data(), h->bt_state.p, bt.size(), hipMemcpyDeviceToHost, s));
+    }
     RC_HIP(hipStreamSynchronize(s));
     float ms_setup = 0.f, ms_pcg = 0.f;
     (void)hipEventElapsedTime(&ms_setup, e0, e1);
@@ -1390,8 +1553,45 @@ static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
         st->setup_ms = ms_setup;
         st->err_vs_oracle = -1.0;
     }
+    bool any_brk = false;
+    if (h->nsys > 0) {
+        // per-system results; the summary reports the worst system
+        BatchArgs hb;
+        hb.nsys = h->nsys;
+        hb.nblk = h->nblk;
+        batch_carve(hb, bt.data());
+        h->bstats.assign((size_t)h->nsys, rc_solve_stats{});
+        int max_it = 0, all_conv = 1;
+        double worst_true = 0.0, worst_pre = 0.0;
+        for (int q = 0; q < h->nsys; ++q) {
+            rc_solve_stats &o = h->bstats[(size_t)q];
+            o.iterations = hb.iter[q];
+            const double pre = hb.rz0[q] > 0.0 ? std::sqrt(std::max(hb.rz[q], 0.0)) / std::sqrt(hb.rz0[q]) : 0.0;
+            const double tr = hb.bn2[q] > 0.0 ? std::sqrt(hb.res2[q]) / std::sqrt(hb.bn2[q]) : 0.0;
+            const bool met = h->opt.stop_rule == RC_STOP_REFERENCE
+                                 ? (hb.rz0[q] <= tol || pre <= tol)
+                                 : (hb.bb[q] <= 0.0 || std::sqrt(hb.rr[q]) / std::sqrt(hb.bb[q]) <= tol);
+            o.converged = (sc.status == 0 && !hb.brk[q] && hb.done[q] && met) ? 1 : 0;
+            o.rel_res_true = tr;
+            o.rel_res_precond = pre;
+            o.pcg_ms = ms_pcg;
+            o.setup_ms = ms_setup;
+            o.err_vs_oracle = -1.0;
+            any_brk = any_brk || hb.brk[q];
+            max_it = std::max(max_it, o.iterations);
+            all_conv &= o.converged;
+            worst_true = std::max(worst_true, tr);
+            worst_pre = std::max(worst_pre, pre);
+        }
+        if (st) {
+            st->iterations = max_it;
+            st->converged = all_conv;
+            st->rel_res_true = worst_true;
+            st->rel_res_precond = worst_pre;
+        }
+    }
     if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
-    if (sc.status & kPcgBreakdown) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown (p'Ap <= 0)");
+    if ((sc.status & kPcgBreakdown) || any_brk) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown (p'Ap <= 0)");
     return RC_OK;
 }
 
@@ -1445,6 +1645,27 @@ int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_sta
     return rc;
 }
 
+int rc_solve_batch(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *per_system)
+{
+    RC_CHECK_HANDLE();
+    if (h->nsys <= 0) return fail(h, RC_ERR_STATE, "handle holds no batch (rc_set_batch)");
+    rc_solve_stats total;
+    int rc = rc_solve(h, b, x, tol, &total);
+    if (rc != RC_OK && rc != RC_ERR_BREAKDOWN) return rc;
+    if (per_system && h->bstats.size() == (size_t)h->nsys)
+        std::memcpy(per_system, h->bstats.data(), h->bstats.size() * sizeof(rc_solve_stats));
+    return rc;
+}
+
+int rc_get_batch_stats(rc_handle_t h, rc_solve_stats *per_system)
+{
+    RC_CHECK_HANDLE();
+    if (h->nsys <= 0 || h->bstats.size() != (size_t)h->nsys) return fail(h, RC_ERR_STATE, "no batched solve");
+    if (!per_system) return fail(h, RC_ERR_ARG, "null stats array");
+    std::memcpy(per_system, h->bstats.data(), h->bstats.size() * sizeof(rc_solve_stats));
+    return RC_OK;
+}
+
 // ------------------------------------------------------------------ multi-RHS
 }  // extern "C"
 
@@ -1495,6 +1716,7 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
     if (k <= 0 || k > kMaxRhs || !B || !X || !(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
     if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_solve_multi");
+    if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
     RC_HIP(hipSetDevice(h->opt.device));
     int rc = h->prec == 32 ? build_multi<float>(h) : build_multi<double>(h);
     if (rc != RC_OK) return rc;

@@ -208,3 +208,68 @@ def test_multi_rhs_matches_oracle(rc, orc, wl, k, prec):
         assert np.linalg.norm(X[perm, q] - xo) <= tol * max(np.linalg.norm(xo), 1e-300)
     if k == 5:
         assert sts[3]["iterations"] == 0 and np.all(X[:, 3] == 0.0)
+
+
+# ---------------------------------------------------------------- batched systems
+def _batch_systems(wl, many=False):
+    if many:
+        sizes = [20 + (37 * q) % 280 for q in range(60)]
+        return [wl.random_geometric_graph(m, seed=500 + q) for q, m in enumerate(sizes)]
+    return [wl.grid_laplacian((12, 12), "loguniform", seed=5),
+            wl.random_geometric_graph(2500, seed=9),
+            wl.grid_laplacian((6, 6, 6), "uniform_0.5_2", seed=3),
+            wl.grid_laplacian((30, 20))]
+
+
+def _block_diag(systems):
+    off, R, C, V = 0, [], [], []
+    for (n, r, c, v) in systems:
+        R.append(np.asarray(r) + off)
+        C.append(np.asarray(c) + off)
+        V.append(np.asarray(v))
+        off += n
+    return off, np.concatenate(R), np.concatenate(C), np.concatenate(V)
+
+
+@pytest.mark.parametrize("many", [False, True])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_batch_matches_oracle_per_system(rc, orc, wl, prec, many):
+    """Every system of a batch runs its own PCG: each system's iterate equals
+    the oracle PCG on the block-diagonal system with the other systems'
+    right-hand sides zeroed (which is that system's PCG alone)."""
+    say(f"[gpu] batch prec={prec} many={many}")
+    systems = _batch_systems(wl, many)
+    s = rc.Solver(seed=11, precision=prec)
+    s.set_batch(systems)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    fs = s.factor()
+    k, off = s.batch()
+    assert k == len(systems)
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    zero = 2
+    b[off[zero]:off[zero + 1]] = 0.0          # stops before the loop and stays at x = 0
+    x, sts = s.solve_batch(b, tol=1e-6)
+    n, r, c, v = _block_diag(systems)
+    fac, sys_, perm, _ = orc.prepare_system(n, r, c, v, kind="laplacian", order="nnz-sort", order_seed=1)
+    f = orc.factor(fac, seed=11)
+    assert fs["nnz_factor"] == f.stats["nnz_off"] + n
+    tol = 1e-7 if prec == 64 else 1e-4
+    its = []
+    for q in range(k):
+        bq = np.zeros(n)
+        bq[off[q]:off[q + 1]] = b[off[q]:off[q + 1]]
+        xo, so = orc.pcg(sys_, f, bq[perm], tol=1e-6, precision=prec)
+        xq = np.zeros(n)
+        xq[off[q]:off[q + 1]] = x[off[q]:off[q + 1]]
+        assert abs(sts[q]["iterations"] - so["iterations"]) <= 1, (q, sts[q], so)
+        assert np.linalg.norm(xq[perm] - xo) <= tol * max(np.linalg.norm(xo), 1e-300), q
+        if q != zero:
+            assert sts[q]["converged"] == 1 and sts[q]["rel_res_true"] < 1e-4, (q, sts[q])
+        its.append(sts[q]["iterations"])
+    assert sts[zero]["iterations"] == 0 and np.all(x[off[zero]:off[zero + 1]] == 0.0)
+    assert len(set(its)) > 1                   # systems stop at different iterations
+    # the summary of rc_solve_resident on the same batch is the worst system
+    s.set_rhs(b)
+    tot = s.solve_resident(1e-6)
+    assert tot["iterations"] == max(its)
+    assert s.batch_stats()[1]["iterations"] == sts[1]["iterations"]

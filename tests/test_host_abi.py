@@ -237,3 +237,36 @@ def test_amd_close_to_exact_minimum_degree_on_small_graphs(rc):
         amd = _symbolic_fill(*s.prepared_csc()[:3])
         md = _exact_min_degree_fill(n, edges)
         assert amd <= md + 2, (n, edges, amd, md)
+
+
+def test_batch_input_host(rc, wl):
+    """rc_set_batch: block-diagonal assembly, offsets, per-system RHS
+    projection, and the error paths (no device needed)."""
+    s = rc.Solver(device=-1)
+    sysA = wl.grid_laplacian((4, 5))
+    sysB = wl.grid_laplacian((3, 3, 2), "uniform_0.5_2", seed=2)
+    s.set_batch([sysA, sysB])
+    k, off = s.batch()
+    assert k == 2 and list(off) == [0, 20, 38]
+    n, nnz = s.size()
+    assert n == 38
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    assert abs(b[:20].sum()) < 1e-12 and abs(b[20:].sum()) < 1e-12
+    # the prepared Laplacian is block diagonal
+    s.order(rc.ORDER_AS_IS, 1)
+    nn, ptr, ind, val = s.prepared_csc()
+    for j in range(nn):
+        rows = ind[ptr[j]:ptr[j + 1]]
+        assert np.all((rows < 20) == (j < 20))
+    # a plain handle reports one system
+    t = rc.Solver(device=-1)
+    t.set_triplets(*sysA)
+    assert t.batch()[0] == 1
+    # an index outside its own system is rejected
+    n0, r0, c0, v0 = sysA
+    bad = (n0, np.array(r0), np.array(c0), np.array(v0))
+    bad[1][0] = n0
+    with pytest.raises(rc.RcholError):
+        s.set_batch([bad, sysB])
+    with pytest.raises(rc.RcholError):
+        s.set_batch([(0, np.array([], np.int32), np.array([], np.int32), np.array([]))])
