@@ -254,6 +254,9 @@ def main():
         ms, by = s.time_kernel(kid, reps=args.kernel_reps)
         kern[name] = {"ms": ms, "bytes": by, "GBps": by / ms / 1e6}
         log(f"[bench] kernel {name}: {ms:.4f} ms  {by / 1e6:.1f} MB  {by / ms / 1e6:.1f} GB/s")
+    ms, by = s.time_kernel(rc.KERNEL_COPY, reps=max(3, args.kernel_reps // 2))
+    kern["copy_1GiB"] = {"ms": ms, "bytes": by, "GBps": by / ms / 1e6}
+    log(f"[bench] kernel copy (calibration): {ms:.4f} ms  {by / ms / 1e6:.1f} GB/s")
     fac_gbs = fst["bytes_factor"] / fst["factor_ms"] / 1e6
     kern["factor"] = {"ms": fst["factor_ms"], "bytes": fst["bytes_factor"], "GBps": fac_gbs}
     per_solve = {k: kern[k]["ms"] * (np.mean(iters) + 1) for k in ("spmv", "trsv_fwd", "trsv_bwd")}

@@ -29,7 +29,7 @@ INPUT_LAPLACIAN, INPUT_GROUNDED, INPUT_SDDM = 0, 1, 2
 ORDER_AS_IS, ORDER_NNZ_SORT, ORDER_RANDOM, ORDER_AMD, ORDER_LEARNED = 0, 1, 2, 3, 4
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 RHS_ZERO_SUM_UNIFORM, RHS_NORMAL, RHS_NORMAL_PROJECTED = 0, 1, 2
-KERNEL_SPMV, KERNEL_TRSV_FWD, KERNEL_TRSV_BWD, KERNEL_PCG_ITER = 0, 1, 2, 3
+KERNEL_SPMV, KERNEL_TRSV_FWD, KERNEL_TRSV_BWD, KERNEL_PCG_ITER, KERNEL_COPY = 0, 1, 2, 3, 4
 
 # Every symbol include/rchol.h declares.
 EXPORTS = (

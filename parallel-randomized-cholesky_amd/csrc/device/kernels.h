@@ -207,6 +207,8 @@ void launch_residual(const SolveArgs &a, hipStream_t s);
 void launch_gather(const double *src, const int *idx, double *dst, int n, hipStream_t s);
 void launch_scatter(const double *src, const int *idx, double *dst, int n, hipStream_t s);
 void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s);
+// dst[0..n) = src[0..n), n even
+void launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
 
 // ------------------------------------------------------------ ordering network
 // Weight vector layout (float32): per layer Ws (64 x in), Wn (64 x in), b (64)

@@ -467,6 +467,12 @@ __global__ void k_scatter(const double *src, const int *idx, double *dst, int n)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[idx[i]] = src[i];
 }
+// Plain streaming copy (calibration of bandwidth and of the HBM counters).
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 __global__ void k_reset_counters(unsigned int *a, unsigned int *b)
 {
     if (threadIdx.x == 0) { *a = 0u; *b = 0u; }
@@ -747,6 +753,10 @@ void launch_gather(const double *src, const int *idx, double *dst, int n, hipStr
 void launch_scatter(const double *src, const int *idx, double *dst, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scatter, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
+}
+void launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n / 2);
 }
 void launch_reset_counters(unsigned int *a, unsigned int *b, hipStream_t s)
 {

@@ -111,6 +111,7 @@ struct rc_handle_s {
     // PCG
     DevVec a_ptr, a_ind, a_val;
     DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d;
+    DevVec calib;                   // copy-kernel calibration buffer (rc_time_kernel)
     int reduce_blocks = 1024;
     int sweep_blocks = 1024;
     bool rhs_set = false;
@@ -1831,6 +1832,8 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
 }
 
 // ------------------------------------------------------------------ timing
+static const size_t kCalibBytes = (size_t)1 << 30;   // copy kernel: 1 GiB read + 1 GiB written
+
 int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
 {
     RC_CHECK_HANDLE();
@@ -1888,6 +1891,13 @@ int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, d
             RC_HIP(hipEventRecord(e0, s));
             if (h->prec == 32) launch_pcg_iteration<float>(a, t & 1, s); else launch_pcg_iteration<double>(a, t & 1, s);
             RC_HIP(hipEventRecord(e1, s));
+        } else if (kernel == RC_KERNEL_COPY) {
+            RC_HIP(h->calib.ensure(2 * kCalibBytes));
+            double *src = h->calib.as<double>();
+            if (t == 0) RC_HIP(hipMemsetAsync(src, 0, 2 * kCalibBytes, s));
+            RC_HIP(hipEventRecord(e0, s));
+            launch_copy(src, src + kCalibBytes / 8, kCalibBytes / 8, s);
+            RC_HIP(hipEventRecord(e1, s));
         } else {
             return fail(h, RC_ERR_ARG, "unknown kernel id");
         }
@@ -1904,6 +1914,7 @@ int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, d
     *ms_per_launch = total / reps;
     if (bytes_per_launch) {
         if (kernel == RC_KERNEL_SPMV) *bytes_per_launch = b_spmv;
+        else if (kernel == RC_KERNEL_COPY) *bytes_per_launch = 2.0 * (double)kCalibBytes;
         else if (kernel == RC_KERNEL_PCG_ITER) *bytes_per_launch = b_spmv + 2.0 * b_tri + sz * n + 72.0 * n;
         else *bytes_per_launch = b_tri;
     }
