@@ -263,14 +263,44 @@ __global__ void k_ell_fill(int nslices, const int *start, const int *cnt, const 
     }
 }
 
-__global__ void k_chunk_count(int nlevels, const int *lvl_ptr, int *cnt)
+__global__ void k_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw)
 {
-    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlevels; l += gridDim.x * blockDim.x)
-        cnt[l] = (lvl_ptr[l + 1] - lvl_ptr[l] + kWave - 1) / kWave;
+    const int first = (blockIdx.x * blockDim.x + threadIdx.x) & ~(kWave - 1);
+    for (int base = first; base < ns; base += gridDim.x * blockDim.x) {
+        // the loop bound is uniform over the wave; lanes past ns count as
+        // width 0 at level -1
+        const int p = base + lane_id();
+        int L = -1, w = 0;
+        if (p < ns) {
+            const int row = order[p];
+            L = level[row];
+            const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
+            w = a > b ? a : b;
+        }
+        const int L0 = __shfl(L, 0, kWave);
+        const bool uniform = __all(L == L0 || L < 0);
+        if (uniform) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const int o = __shfl_xor(w, off, kWave);
+                w = o > w ? o : w;
+            }
+            if (lane_id() == 0 && L0 >= 0) atomicMax(lw + L0, w);
+        } else if (L >= 0) {
+            atomicMax(lw + L, w);
+        }
+    }
 }
 
-__global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, int *chunk_start,
-                             int *chunk_cnt)
+__global__ void k_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt)
+{
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlevels; l += gridDim.x * blockDim.x) {
+        const int r = chunk_rows(lw[l]);
+        cnt[l] = (lvl_ptr[l + 1] - lvl_ptr[l] + r - 1) / r;
+    }
+}
+
+__global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
+                             int *chunk_start, int *chunk_cnt)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
         int lo = 0, hi = nlevels - 1;          // last level with chunk_ptr[l] <= c
@@ -278,12 +308,145 @@ __global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const
             int mid = (lo + hi + 1) >> 1;
             if (chunk_ptr[mid] <= c) lo = mid; else hi = mid - 1;
         }
-        int start = lvl_ptr[lo] + (c - chunk_ptr[lo]) * kWave;
+        const int r = chunk_rows(lw[lo]);
+        int start = lvl_ptr[lo] + (c - chunk_ptr[lo]) * r;
         int end = lvl_ptr[lo + 1];
         chunk_start[c] = start;
-        chunk_cnt[c] = (end - start) < kWave ? (end - start) : kWave;
+        chunk_cnt[c] = (end - start) < r ? (end - start) : r;
     }
 }
+
+__global__ void k_pack_meta(int nchunks, const int *start, const int *cnt, const int *off, const int *slices,
+                            int4 *meta)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x)
+        meta[c] = make_int4(start[c], cnt[c], off[c], slices[c]);
+}
+// ---------------------------------------------------------------- tail split
+__device__ __forceinline__ int src_entry(const EllSrc &src, int b, int et, int &pos)
+{
+    const int e = src.eidx ? src.eidx[b + et] : b + et;
+    const int col = src.rowk ? src.rowk[e] : src.ind[e];
+    pos = src.colmap[col];
+    return e;
+}
+
+// One wave per tail row: entries with column position < tp0 and >= tp0.
+__global__ void k_tail_count(int ntail, int tp0, const int *order, EllSrc src, int *cnt_b, int *cnt_i)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int q = wave; q < ntail; q += nwaves) {
+        const int row = order[tp0 + q];
+        const int b = src.ptr[row], len = src.ptr[row + 1] - b;
+        int nb = 0;
+        for (int t = lane; t < len; t += kWave) {
+            int pos;
+            (void)src_entry(src, b, t, pos);
+            nb += pos < tp0 ? 1 : 0;
+        }
+        for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off, kWave);
+        if (lane == 0) {
+            cnt_b[q] = nb;
+            cnt_i[q] = len - nb;
+        }
+    }
+}
+
+// One wave per tail row: stable compaction of the two parts (ballot ranks).
+template <typename TD>
+__global__ void k_tail_split(int ntail, int tp0, const int *order, EllSrc src, const int *ptr_b, const int *ptr_i,
+                             int *ind_b, TD *val_b, int *ind_i, TD *val_i)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    const TD *sval = (const TD *)src.val;
+    for (int q = wave; q < ntail; q += nwaves) {
+        const int row = order[tp0 + q];
+        const int b = src.ptr[row], len = src.ptr[row + 1] - b;
+        int ob = ptr_b[q], oi = ptr_i[q];
+        for (int base = 0; base < len; base += kWave) {
+            const int t = base + lane;
+            int pos = 0, e = 0;
+            if (t < len) e = src_entry(src, b, t, pos);
+            const bool isb = t < len && pos < tp0;
+            const bool isi = t < len && pos >= tp0;
+            const uint64_t mb = ballot(isb), mi = ballot(isi);
+            if (isb) {
+                const int o = ob + lanes_below(mb);
+                ind_b[o] = pos;
+                val_b[o] = sval[e];
+            }
+            if (isi) {
+                const int o = oi + lanes_below(mi);
+                ind_i[o] = pos - tp0;
+                val_i[o] = sval[e];
+            }
+            ob += __popcll(mb);
+            oi += __popcll(mi);
+        }
+    }
+}
+
+// One wave per tail level: widest internal row -> slices and block size.
+__global__ void k_tail_layout(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, int *slices,
+                              int *size)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int L = wave; L < ntl; L += nwaves) {
+        const int q0 = lstart[L] - tp0, cnt = lcnt[L];
+        int w = 0;
+        for (int i = lane; i < cnt; i += kWave) {
+            const int len = ptr_i[q0 + i + 1] - ptr_i[q0 + i];
+            w = len > w ? len : w;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(w, off, kWave);
+            w = o > w ? o : w;
+        }
+        if (lane == 0) {
+            const int G = tail_group(cnt);
+            const int sl = (w + G - 1) / G;
+            slices[L] = sl;
+            size[L] = sl * sweep_rows_pow2(cnt) * G;
+        }
+    }
+}
+
+template <typename TD>
+__global__ void k_tail_fill(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, const int *ind_i,
+                            const TD *val_i, const int *off, const int *slices, int pad_ind, int *ell_ind, TD *ell_val)
+{
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int L = wave; L < ntl; L += nwaves) {
+        const int cnt = lcnt[L], q0 = lstart[L] - tp0;
+        const int P = sweep_rows_pow2(cnt), G = tail_group(cnt);
+        const int stride = P * G, total = slices[L] * stride, o = off[L];
+        for (int idx = lane; idx < total; idx += kWave) {
+            const int sl = idx / stride, t = idx - sl * stride;
+            const int i = t / G, g = t - i * G;
+            const int et = sl * G + g;
+            int ind = pad_ind;
+            TD v = (TD)0;
+            if (i < cnt) {
+                const int b = ptr_i[q0 + i];
+                if (et < ptr_i[q0 + i + 1] - b) {
+                    ind = ind_i[b + et];
+                    v = val_i[b + et];
+                }
+            }
+            ell_ind[o + idx] = ind;
+            ell_val[o + idx] = v;
+        }
+    }
+}
+
 __global__ void k_sweep_layout(int nchunks, const int *cnt, const int *width, int *slices, int *size)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
@@ -429,16 +592,66 @@ void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *o
                        width, pad_ind, pad_val, ell_ind, ell_val);
 }
 
-void launch_chunk_count(int nlevels, const int *lvl_ptr, int *cnt, hipStream_t s)
+void launch_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw,
+                        hipStream_t s)
 {
-    hipLaunchKernelGGL(k_chunk_count, dim3(256), dim3(256), 0, s, nlevels, lvl_ptr, cnt);
+    hipLaunchKernelGGL(k_level_width, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, lw);
 }
-void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, int *chunk_start,
-                       int *chunk_cnt, hipStream_t s)
+void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_chunk_fill, dim3(1024), dim3(256), 0, s, nchunks, nlevels, lvl_ptr, chunk_ptr, chunk_start,
-                       chunk_cnt);
+    hipLaunchKernelGGL(k_chunk_count, dim3(256), dim3(256), 0, s, nlevels, lvl_ptr, lw, cnt);
 }
+void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
+                       int *chunk_start, int *chunk_cnt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_chunk_fill, dim3(1024), dim3(256), 0, s, nchunks, nlevels, lvl_ptr, chunk_ptr, lw,
+                       chunk_start, chunk_cnt);
+}
+void launch_pack_meta(int nchunks, const int *start, const int *cnt, const int *off, const int *slices, int4 *meta,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_meta, dim3(1024), dim3(256), 0, s, nchunks, start, cnt, off, slices, meta);
+}
+
+static int waves_grid(int items)
+{
+    int g = (items + 3) / 4;
+    if (g > 16384) g = 16384;
+    return g < 1 ? 1 : g;
+}
+void launch_tail_count(int ntail, int tp0, const int *order, EllSrc src, int *cnt_b, int *cnt_i, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tail_count, dim3(waves_grid(ntail)), dim3(256), 0, s, ntail, tp0, order, src, cnt_b, cnt_i);
+}
+template <typename TD>
+void launch_tail_split(int ntail, int tp0, const int *order, EllSrc src, const int *ptr_b, const int *ptr_i,
+                       int *ind_b, TD *val_b, int *ind_i, TD *val_i, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tail_split<TD>, dim3(waves_grid(ntail)), dim3(256), 0, s, ntail, tp0, order, src, ptr_b,
+                       ptr_i, ind_b, val_b, ind_i, val_i);
+}
+void launch_tail_layout(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, int *slices,
+                        int *size, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tail_layout, dim3(waves_grid(ntl)), dim3(256), 0, s, ntl, lstart, lcnt, tp0, ptr_i, slices,
+                       size);
+}
+template <typename TD>
+void launch_tail_fill(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, const int *ind_i,
+                      const TD *val_i, const int *off, const int *slices, int pad_ind, int *ell_ind, TD *ell_val,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tail_fill<TD>, dim3(waves_grid(ntl)), dim3(256), 0, s, ntl, lstart, lcnt, tp0, ptr_i, ind_i,
+                       val_i, off, slices, pad_ind, ell_ind, ell_val);
+}
+template void launch_tail_split<float>(int, int, const int *, EllSrc, const int *, const int *, int *, float *, int *,
+                                       float *, hipStream_t);
+template void launch_tail_split<double>(int, int, const int *, EllSrc, const int *, const int *, int *, double *,
+                                        int *, double *, hipStream_t);
+template void launch_tail_fill<float>(int, const int *, const int *, int, const int *, const int *, const float *,
+                                      const int *, const int *, int, int *, float *, hipStream_t);
+template void launch_tail_fill<double>(int, const int *, const int *, int, const int *, const int *, const double *,
+                                       const int *, const int *, int, int *, double *, hipStream_t);
 
 template void launch_lt_fill<float>(const AssembleArgs &, const int *, int *, float *, float *, hipStream_t);
 template void launch_lt_fill<double>(const AssembleArgs &, const int *, int *, double *, double *, hipStream_t);

@@ -7,6 +7,8 @@
 
 namespace rc {
 
+constexpr int kWaveRows256 = 256;   // threads of a sweep chunk group
+
 // ------------------------------------------------------------ elimination
 struct FillRec {            // one pending fill edge, 16 B
     int row;
@@ -98,9 +100,25 @@ void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int
 void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s);
 void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s);
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s);
-void launch_chunk_count(int nlevels, const int *lvl_ptr, int *cnt, hipStream_t s);
-void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, int *chunk_start,
-                       int *chunk_cnt, hipStream_t s);
+// Rows per sweep chunk of a level whose widest row (of L and of L^T) has
+// `width` entries: G = next power of two >= width / 8 lanes per row (4..64),
+// so a lane holds at most eight entries of a row, and 256 / G rows.
+__host__ __device__ inline int chunk_rows(int width)
+{
+    int g = 4;
+    while (g < 64 && g * 8 < width) g <<= 1;
+    return kWaveRows256 / g;
+}
+// lw[L] = max over the rows of level L of max(len L row, len L^T row)
+// (lw zeroed by the caller); rows at positions p, level of p = level[order[p]].
+void launch_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw,
+                        hipStream_t s);
+void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt, hipStream_t s);
+void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
+                       int *chunk_start, int *chunk_cnt, hipStream_t s);
+// meta[c] = {start[c], cnt[c], off[c], slices[c]}
+void launch_pack_meta(int nchunks, const int *start, const int *cnt, const int *off, const int *slices, int4 *meta,
+                      hipStream_t s);
 template <typename T> void launch_gather_t(const T *src, const int *idx, T *dst, int n, hipStream_t s);
 void launch_gather_i(const int *src, const int *idx, int *dst, int n, hipStream_t s);
 
@@ -146,6 +164,33 @@ __host__ __device__ inline int sweep_group(int cnt)
     return g > 64 ? 64 : g;
 }
 void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *slices, int *size, hipStream_t s);
+
+// Tail of the level order: the rows at positions [tp0, ns) (the narrow
+// trailing levels) are solved by one kTailThreads workgroup whose LDS holds
+// their values.  The entries of each tail row's source row (a row of L for the
+// forward sweep, of L^T for the backward sweep) are split by column position:
+// below tp0 ("bulk part": final before the tail starts, folded in by a wide
+// kernel) or at/after tp0 ("internal": read from LDS, column stored as
+// position - tp0).  A tail level of cnt <= kTailMaxRows rows is one group
+// with tail_group(cnt) lanes per row: entry t of level row i lives at
+// off[L] + (t / G) * P * G + i * G + t % G (P = next power of two >= cnt).
+constexpr int kTailMaxRows = 1024;
+__host__ __device__ inline int tail_group(int cnt)
+{
+    int g = kTailThreads / sweep_rows_pow2(cnt);
+    return g > 64 ? 64 : (g < 1 ? 1 : g);
+}
+void launch_tail_count(int ntail, int tp0, const int *order, EllSrc src, int *cnt_b, int *cnt_i, hipStream_t s);
+template <typename TD>
+void launch_tail_split(int ntail, int tp0, const int *order, EllSrc src, const int *ptr_b, const int *ptr_i,
+                       int *ind_b, TD *val_b, int *ind_i, TD *val_i, hipStream_t s);
+// slices[L] = ceil(max internal row length of level L / G), size[L] = slices * P * G
+void launch_tail_layout(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, int *slices,
+                        int *size, hipStream_t s);
+template <typename TD>
+void launch_tail_fill(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, const int *ind_i,
+                      const TD *val_i, const int *off, const int *slices, int pad_ind, int *ell_ind, TD *ell_val,
+                      hipStream_t s);
 template <typename TS, typename TD>
 void launch_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src, const int *off,
                        const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val, hipStream_t s);
@@ -169,6 +214,7 @@ struct SolveArgs {
     // level schedule (positions)
     int nchunks;
     const int *chunk_start, *chunk_cnt;
+    const int4 *cm_f, *cm_b;      // per chunk {start, cnt, ELL offset, slices} of the two sweeps
     // L rows (forward) and L^T rows (backward) in grouped sliced ELL per chunk
     // (lf_width / lb_width hold the slice counts)
     const int *lf_off, *lf_width, *lf_ind;
@@ -183,11 +229,21 @@ struct SolveArgs {
     int reduce_blocks;
     int sweep_blocks;             // persistent sweep workgroups (occupancy-sized)
     // level split: chunks [0, bulk_chunks) belong to the wide levels and run
-    // on the persistent grid; levels [tail_level, nlevels) run in one
-    // workgroup, level by level (chunk range of level L: tail_cptr[L] ..
-    // tail_cptr[L + 1], absolute chunk indices)
+    // on the persistent grid; levels [tail_level, nlevels) form the tail
     int bulk_chunks, tail_level, nlevels;
-    const int *tail_cptr;
+    // tail (see launch_tail_*): rows [tail_pos0, n), ntail_levels levels with
+    // absolute start positions tl_start and row counts tl_cnt; internal ELL of
+    // the forward (tf_*) and backward (tb_*) sweeps; CSR (tp_*) of the
+    // forward bulk part (absolute column positions); tacc = bulk-folded
+    // right-hand side of the tail rows (precision of the factor)
+    int tail_pos0, ntail, ntail_levels, tail_lds_bytes;
+    const int *tl_start, *tl_cnt;
+    const int *tf_off, *tf_slices, *tf_ind;
+    const int *tb_off, *tb_slices, *tb_ind;
+    const void *tf_val, *tb_val;
+    const int *tp_ptr, *tp_ind;
+    const void *tp_val;
+    void *tacc;
     long long spin_ticks;
     const double *b;
     // batched systems: system of every position and the per-system alpha
@@ -196,6 +252,8 @@ struct SolveArgs {
     const double *alpha_sys;
 };
 int sweep_occupancy_blocks(int device);
+// Largest dynamic LDS (bytes) the tail workgroup may use on this device.
+int tail_lds_limit(int device);
 
 template <typename T> void launch_fill_sentinel(T *v, int n, hipStream_t s);
 template <typename T> void launch_pcg_init(const SolveArgs &a, hipStream_t s);

@@ -217,119 +217,267 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const dou
 }
 
 // ---------------------------------------------------------------- K2/K3
-// Chunk c of the forward sweep, processed by one 256-thread group (thread t
-// of the group); G lanes per row.
+// Chunk m = {start, cnt, ELL offset, slices} of the forward sweep, processed
+// by one 256-thread group (thread t of the group); G lanes per row.  The row's
+// own values and the chunk's index/value slices are loaded together, then
+// the dependencies.
 template <typename T>
-__device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int c, int t, double alpha, const double *p_cur,
+__device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, double alpha, const double *p_cur,
                                           int update_rx)
 {
     const T *lval = (const T *)a.lf_val;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
-    const int cnt = a.chunk_cnt[c];
+    const int cnt = m.y;
     const int G = sweep_group(cnt);
     const int stride = sweep_rows_pow2(cnt) * G;
     const int i = t / G, g = t - i * G;
     const bool active = t < stride && i < cnt;
-    const int k = a.chunk_start[c] + i;
+    const bool lead = active && g == 0;
+    const int k = m.x + i;
     T acc = (T)0;
-    if (active && g == 0) {
-        double rk = a.r[k];
+    double rk = 0.0, apk = 0.0, xk = 0.0, pk = 0.0;
+    if (lead) {
+        rk = a.r[k];
         if (update_rx) {
+            apk = a.ap[k];
+            xk = a.x[k];
+            pk = p_cur[k];
             if (a.alpha_sys) alpha = a.alpha_sys[a.sys_of[k]];
-            rk = fma_t(-alpha, a.ap[k], rk);
-            a.r[k] = rk;
-            a.x[k] = fma_t(alpha, p_cur[k], a.x[k]);
         }
-        acc = (T)rk;
     }
     if (t < stride) {
-        const int off = a.lf_off[c] + t;
-        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, a.lf_width[c], stride, y, acc, a.spin_ticks, a.sc);
+        const int off = m.z + t;
+        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, m.w, stride, y, acc, a.spin_ticks, a.sc);
+    }
+    if (lead) {
+        if (update_rx) {
+            rk = fma_t(-alpha, apk, rk);
+            a.r[k] = rk;
+            a.x[k] = fma_t(alpha, pk, xk);
+        }
+        acc += (T)rk;
     }
     acc = group_sum(acc, G);
-    if (active && g == 0) {
+    if (lead) {
         st_relaxed_val(y + k, unsent(acc));
         z[k] = sent_value(T());
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int c, int t)
+__device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
 {
     const T *ltval = (const T *)a.lb_val;
     const T *diag = (const T *)a.diag;
     T *y = (T *)a.y;
     T *z = (T *)a.z;
-    const int cnt = a.chunk_cnt[c];
+    const int cnt = m.y;
     const int G = sweep_group(cnt);
     const int stride = sweep_rows_pow2(cnt) * G;
     const int i = t / G, g = t - i * G;
     const bool active = t < stride && i < cnt;
-    const int k = a.chunk_start[c] + i;
-    T acc = (T)0;
-    if (active && g == 0) {
-        T yk = y[k];
-        y[k] = sent_value(T());
-        T dk = diag[k];
-        if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
-        acc = yk;
+    const bool lead = active && g == 0;
+    const int k = m.x + i;
+    T acc = (T)0, yk = (T)0, dk = (T)0;
+    if (lead) {
+        yk = y[k];
+        dk = diag[k];
     }
     if (t < stride) {
-        const int off = a.lb_off[c] + t;
-        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, a.lb_width[c], stride, z, acc, a.spin_ticks, a.sc);
+        const int off = m.z + t;
+        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, m.w, stride, z, acc, a.spin_ticks, a.sc);
+    }
+    if (lead) {
+        y[k] = sent_value(T());
+        if (dk != (T)0) yk = yk / dk;            // solver.hpp:64-84
+        acc += yk;
     }
     acc = group_sum(acc, G);
-    if (active && g == 0) st_relaxed_val(z + k, unsent(acc));
+    if (lead) st_relaxed_val(z + k, unsent(acc));
 }
 
 // Persistent part: workgroups claim chunks of the wide levels in order
 // (forward) or reverse order (backward); dependencies are awaited per value.
+// Thread 0 holds the claim of the next chunk while the current one is
+// processed, and fetches the chunk's packed descriptor.
 template <typename T, bool kFwd>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const double *p_cur, int update_rx)
 {
-    __shared__ int s_chunk, s_stop;
+    __shared__ int4 s_meta;
+    __shared__ int s_go;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
     unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
+    const int4 *meta = kFwd ? a.cm_f : a.cm_b;
     const int t = threadIdx.x;
+    int next = 0;
+    if (t == 0) next = (int)atomicAdd(ctr, 1u);
     while (true) {
         if (t == 0) {
-            s_chunk = (int)atomicAdd(ctr, 1u);
-            s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
+            const unsigned int st = ld_relaxed(&sc->status);
+            const int c = next;
+            const bool go = c < a.bulk_chunks;
+            int4 m = make_int4(0, 0, 0, 0);
+            if (go) m = meta[kFwd ? c : a.bulk_chunks - 1 - c];
+            if (go) next = (int)atomicAdd(ctr, 1u);
+            s_meta = m;
+            s_go = (go && !(st & kPcgTimeout)) ? 1 : 0;
         }
         __syncthreads();
-        int c = s_chunk;
-        const int stop = s_stop;
+        const int4 m = s_meta;
+        const int go = s_go;
         __syncthreads();
-        if (c >= a.bulk_chunks || stop) break;
-        if (kFwd) fwd_chunk<T>(a, c, t, alpha, p_cur, update_rx);
-        else bwd_chunk<T>(a, a.bulk_chunks - 1 - c, t);
+        if (!go) break;
+        if (kFwd) fwd_chunk<T>(a, m, t, alpha, p_cur, update_rx);
+        else bwd_chunk<T>(a, m, t);
     }
 }
 
-// Tail part: the narrow levels, one 1024-thread workgroup, level by level;
-// its four 256-thread groups share the chunks of a level, and a barrier
-// separates levels, so nothing is polled.
-template <typename T, bool kFwd>
-__global__ void __launch_bounds__(kTailThreads) k_sweep_tail(SolveArgs a, const double *p_cur, int update_rx)
+// ---------------------------------------------------------------- tail
+// The narrow trailing levels (rows [tail_pos0, n)) run in one 1024-thread
+// workgroup whose LDS holds the tail's solution values, so the level-to-level
+// dependency chain goes through LDS and a barrier, not through HBM.
+// Forward: k_tail_prep folds every tail row's entries in the wide levels
+// (final once the bulk sweep has ended) into tacc with one wave per row, and
+// applies the row's r/x update; k_tail_solve then walks the tail levels.
+// Backward: the tail levels come first and depend only on each other.
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_tail_prep(SolveArgs a, const double *p_cur, int update_rx)
 {
     PcgScalars *sc = a.sc;
     if (sc->done) return;
-    const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
-    const int grp = threadIdx.x / kSweepThreads;
-    const int t = threadIdx.x - grp * kSweepThreads;
-    const int ngrp = kTailThreads / kSweepThreads;
-    const int ntail = a.nlevels - a.tail_level;
-    for (int q = 0; q < ntail; ++q) {
-        const int L = kFwd ? a.tail_level + q : a.nlevels - 1 - q;
-        const int c0 = a.tail_cptr[L], c1 = a.tail_cptr[L + 1];
-        for (int c = c0 + grp; c < c1; c += ngrp) {
-            if (kFwd) fwd_chunk<T>(a, c, t, alpha, p_cur, update_rx);
-            else bwd_chunk<T>(a, c, t);
+    const double alpha0 = update_rx ? sc->alpha : 0.0;
+    const T *val = (const T *)a.tp_val;
+    const T *y = (const T *)a.y;
+    T *tacc = (T *)a.tacc;
+    const int lane = lane_id();
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int q = wave; q < a.ntail; q += nwaves) {
+        T acc = (T)0;
+        for (int e = a.tp_ptr[q] + lane; e < a.tp_ptr[q + 1]; e += kWave) acc = fma_t(-val[e], y[a.tp_ind[e]], acc);
+        acc = group_sum(acc, kWave);
+        if (lane == 0) {
+            const int k = a.tail_pos0 + q;
+            double rk = a.r[k];
+            if (update_rx) {
+                const double al = a.alpha_sys ? a.alpha_sys[a.sys_of[k]] : alpha0;
+                rk = fma_t(-al, a.ap[k], rk);
+                a.r[k] = rk;
+                a.x[k] = fma_t(al, p_cur[k], a.x[k]);
+            }
+            tacc[q] = (T)rk + acc;
         }
-        __syncthreads();
+    }
+}
+
+// Waits for this wave's LDS operations, then the workgroup barrier; global
+// loads in flight (the next level's indices) are not waited for.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+constexpr int kTailPref = 8;    // slices per lane loaded before the LDS reads
+
+template <typename T, bool kFwd>
+__global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
+{
+    extern __shared__ double tail_lds_d[];
+    T *v = reinterpret_cast<T *>(tail_lds_d);
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const int t = threadIdx.x;
+    const int nt = a.ntail, tp0 = a.tail_pos0;
+    T *y = (T *)a.y;
+    T *z = (T *)a.z;
+    const int *off = kFwd ? a.tf_off : a.tb_off;
+    const int *slices = kFwd ? a.tf_slices : a.tb_slices;
+    const int *ind = kFwd ? a.tf_ind : a.tb_ind;
+    const T *val = (const T *)(kFwd ? a.tf_val : a.tb_val);
+    // initial values: forward = bulk-folded right-hand side; backward = D+ y
+    // (solver.hpp:64-84), with y reset to the sentinel for the next sweep
+    for (int q = t; q < nt; q += kTailThreads) {
+        T init;
+        if (kFwd) {
+            init = ((const T *)a.tacc)[q];
+        } else {
+            const int k = tp0 + q;
+            T yk = y[k];
+            y[k] = sent_value(T());
+            const T dk = ((const T *)a.diag)[k];
+            if (dk != (T)0) yk = yk / dk;
+            init = yk;
+        }
+        v[q] = init;
+    }
+    if (t == 0) v[nt] = (T)0;        // slot of the padding entries
+    __syncthreads();
+    // Software pipeline: the descriptor and this lane's first kTailPref
+    // index/value slices of level s + 1 are loaded while level s is solved.
+    const int ntl = a.ntail_levels;
+    int cur_cnt = 0, cur_start = 0, cur_off = 0, cur_nsl = 0;
+    int cj[kTailPref];
+    T cl[kTailPref];
+    auto fetch = [&](int s, int &cnt, int &start, int &o, int &nsl, int *j, T *l) {
+        const int L = kFwd ? s : ntl - 1 - s;
+        cnt = a.tl_cnt[L];
+        start = a.tl_start[L];
+        o = off[L];
+        nsl = slices[L];
+        const int stride = sweep_rows_pow2(cnt) * tail_group(cnt);
+#pragma unroll
+        for (int u = 0; u < kTailPref; ++u) {
+            j[u] = nt;
+            l[u] = (T)0;
+            if (t < stride && u < nsl) {
+                j[u] = ind[o + t + u * stride];
+                l[u] = val[o + t + u * stride];
+            }
+        }
+    };
+    if (ntl > 0) fetch(0, cur_cnt, cur_start, cur_off, cur_nsl, cj, cl);
+    for (int s = 0; s < ntl; ++s) {
+        int nx_cnt = 0, nx_start = 0, nx_off = 0, nx_nsl = 0;
+        int nj[kTailPref];
+        T nl[kTailPref];
+        if (s + 1 < ntl) fetch(s + 1, nx_cnt, nx_start, nx_off, nx_nsl, nj, nl);
+        const int G = tail_group(cur_cnt);
+        const int stride = sweep_rows_pow2(cur_cnt) * G;
+        const int i = t / G, g = t - i * G;
+        const bool lead = t < stride && i < cur_cnt && g == 0;
+        const int q = cur_start - tp0 + i;
+        T acc = (T)0;
+#pragma unroll
+        for (int u = 0; u < kTailPref; ++u) acc = fma_t(-cl[u], v[cj[u]], acc);
+        if (t < stride) {
+            for (int sl = kTailPref; sl < cur_nsl; ++sl)
+                acc = fma_t(-val[cur_off + t + sl * stride], v[ind[cur_off + t + sl * stride]], acc);
+        }
+        if (lead) acc += v[q];
+        acc = group_sum(acc, G);
+        if (lead) v[q] = acc;
+        lds_barrier();
+        cur_cnt = nx_cnt;
+        cur_start = nx_start;
+        cur_off = nx_off;
+        cur_nsl = nx_nsl;
+#pragma unroll
+        for (int u = 0; u < kTailPref; ++u) {
+            cj[u] = nj[u];
+            cl[u] = nl[u];
+        }
+    }
+    for (int q = t; q < nt; q += kTailThreads) {
+        const int k = tp0 + q;
+        if (kFwd) {
+            y[k] = unsent(v[q]);
+            z[k] = sent_value(T());
+        } else {
+            z[k] = unsent(v[q]);
+        }
     }
 }
 
@@ -490,15 +638,19 @@ void launch_fwd(const SolveArgs &a, const double *p_cur, int update_rx, hipStrea
     if (a.bulk_chunks > 0)
         hipLaunchKernelGGL((k_sweep_bulk<T, true>), dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a, p_cur,
                            update_rx);
-    if (a.tail_level < a.nlevels)
-        hipLaunchKernelGGL((k_sweep_tail<T, true>), dim3(1), dim3(kTailThreads), 0, s, a, p_cur, update_rx);
+    if (a.ntail > 0) {
+        int g = (a.ntail + 3) / 4;
+        if (g > 4096) g = 4096;
+        hipLaunchKernelGGL(k_tail_prep<T>, dim3(g), dim3(kRedBlock), 0, s, a, p_cur, update_rx);
+        hipLaunchKernelGGL((k_tail_solve<T, true>), dim3(1), dim3(kTailThreads), a.tail_lds_bytes, s, a);
+    }
 }
 
 template <typename T>
 void launch_bwd(const SolveArgs &a, hipStream_t s)
 {
-    if (a.tail_level < a.nlevels)
-        hipLaunchKernelGGL((k_sweep_tail<T, false>), dim3(1), dim3(kTailThreads), 0, s, a, (const double *)nullptr, 0);
+    if (a.ntail > 0)
+        hipLaunchKernelGGL((k_tail_solve<T, false>), dim3(1), dim3(kTailThreads), a.tail_lds_bytes, s, a);
     if (a.bulk_chunks > 0)
         hipLaunchKernelGGL((k_sweep_bulk<T, false>), dim3(sweep_blocks(a)), dim3(kSweepThreads), 0, s, a,
                            (const double *)nullptr, 0);
@@ -728,6 +880,20 @@ template <typename T>
 void launch_trsv_bwd_only(const SolveArgs &a, hipStream_t s)
 {
     launch_bwd<T>(a, s);
+}
+
+int tail_lds_limit(int device)
+{
+    int lim = 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lim <= 0)
+        lim = 64 * 1024;
+    if (lim > 160 * 1024) lim = 160 * 1024;
+    (void)hipFuncSetAttribute((const void *)k_tail_solve<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_tail_solve<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_tail_solve<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_tail_solve<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    return lim;
 }
 
 int sweep_occupancy_blocks(int device)

@@ -97,7 +97,7 @@ struct rc_handle_s {
     int prec = 64;
     DevVec lt_ptr, lt_ind, lt_val, diagT;
     // solve layout: level-order positions, sliced ELL of L, L^T and A
-    DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP;
+    DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP, lvl_w, cm_f, cm_b;
     DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
     DevVec a_off, a_width, a_ell_ind, a_ell_val;
     DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
@@ -105,7 +105,11 @@ struct rc_handle_s {
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
     bool spatial_levels = false;
-    DevVec tail_cptr;
+    // tail of the level order (LDS-resident solve)
+    int tail_max_rows = 256, tail_lds_cap = -1;
+    int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
+    DevVec tl_start, tl_cnt, tp_ptr, tp_ind, tp_val, tf_off, tf_slices, tf_ind, tf_val, tb_off, tb_slices, tb_ind,
+        tb_val, ti_ptr, ti_ind, ti_val, tacc;
     rc_factor_stats fstats{};
 
     // PCG
@@ -243,6 +247,8 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     {
         const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
         h->spatial_levels = lo && std::string(lo) == "spatial";
+        const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
+        if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
     }
     *out = h;
     return RC_OK;
@@ -947,7 +953,12 @@ static int assemble(rc_handle_t h, double *ms_out)
     int *lcnt = lptr + nlsz;
     int *cptr = lcnt + nlsz;
     launch_segment_bounds(ns, k0, len_bits + sys_bits, nl, lptr, s);
-    launch_chunk_count(nl, lptr, lcnt, s);
+    // chunk size per level from its widest row (at most 8 entries per lane)
+    RC_HIP(h->lvl_w.ensure(nlsz * sizeof(int)));
+    RC_HIP(hipMemsetAsync(h->lvl_w.p, 0, nlsz * sizeof(int), s));
+    launch_level_width(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(), h->lt_ptr.as<int>(),
+                       h->lvl_w.as<int>(), s);
+    launch_chunk_count(nl, lptr, h->lvl_w.as<int>(), lcnt, s);
     scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
     int nchunks = 0;
     RC_HIP(hipMemcpyAsync(&nchunks, cptr + nl, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -955,21 +966,26 @@ static int assemble(rc_handle_t h, double *ms_out)
     h->nchunks = nchunks;
     RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
-    launch_chunk_fill(nchunks, nl, lptr, cptr, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
-    // level split: the trailing run of levels with fewer than kTailRows rows
-    // goes to the single-workgroup tail sweep
+    launch_chunk_fill(nchunks, nl, lptr, cptr, h->lvl_w.as<int>(), h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
+    // level split: the trailing run of levels with fewer than tail_max_rows
+    // rows, as long as their rows fit the tail workgroup's LDS, is the tail
+    std::vector<int> hl(nlsz), hc(nlsz);
     {
-        std::vector<int> hl(nlsz), hc(nlsz);
         RC_HIP(hipMemcpyAsync(hl.data(), lptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
         RC_HIP(hipMemcpyAsync(hc.data(), cptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
         RC_HIP(hipStreamSynchronize(s));
-        const int kTailRows = 256;
+        if (h->tail_lds_cap < 0) h->tail_lds_cap = tail_lds_limit(h->opt.device);
+        const int64_t cap_rows = (int64_t)h->tail_lds_cap / (int64_t)sizeof(T) - 2;
         int tl = nl;
-        while (tl > 0 && hl[(size_t)tl] - hl[(size_t)tl - 1] < kTailRows) --tl;
+        int64_t rows = 0;
+        while (tl > 0) {
+            const int c = hl[(size_t)tl] - hl[(size_t)tl - 1];
+            if (c >= h->tail_max_rows || rows + c > cap_rows) break;
+            rows += c;
+            --tl;
+        }
         h->tail_level = tl;
         h->bulk_chunks = nl > 0 ? hc[(size_t)tl] : 0;
-        RC_HIP(h->tail_cptr.ensure(nlsz * sizeof(int)));
-        RC_HIP(hipMemcpyAsync(h->tail_cptr.p, cptr, nlsz * sizeof(int), hipMemcpyDeviceToDevice, s));
     }
 
     // 4. sliced ELL of L (forward) and L^T (backward) over the chunks
@@ -1017,6 +1033,83 @@ static int assemble(rc_handle_t h, double *ms_out)
                             s);
     RC_HIP(h->diagP.ensure((size_t)std::max(ns, 1) * sizeof(T)));
     launch_gather_t<T>(h->diagT.as<T>(), h->order.as<int>(), h->diagP.as<T>(), ns, s);
+    RC_HIP(h->cm_f.ensure((size_t)std::max(nchunks, 1) * sizeof(int4)));
+    RC_HIP(h->cm_b.ensure((size_t)std::max(nchunks, 1) * sizeof(int4)));
+    launch_pack_meta(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->lf_off.as<int>(),
+                     h->lf_width.as<int>(), h->cm_f.as<int4>(), s);
+    launch_pack_meta(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->lb_off.as<int>(),
+                     h->lb_width.as<int>(), h->cm_b.as<int4>(), s);
+
+    // 4b. the tail: level table, forward bulk-part CSR, internal ELL per direction
+    h->ntail_levels = nl - h->tail_level;
+    h->tail_pos0 = nl > 0 ? hl[(size_t)h->tail_level] : ns;
+    h->ntail = ns - h->tail_pos0;
+    h->tail_lds_bytes = (int)((((size_t)h->ntail + 1) * sizeof(T) + 15) / 16 * 16);
+    if (h->ntail > 0) {
+        const int ntl = h->ntail_levels, tp0 = h->tail_pos0, nt = h->ntail;
+        std::vector<int> ls((size_t)ntl), lc((size_t)ntl);
+        for (int L = 0; L < ntl; ++L) {
+            ls[(size_t)L] = hl[(size_t)(h->tail_level + L)];
+            lc[(size_t)L] = hl[(size_t)(h->tail_level + L + 1)] - ls[(size_t)L];
+        }
+        RC_HIP(h->tl_start.ensure((size_t)ntl * sizeof(int)));
+        RC_HIP(h->tl_cnt.ensure((size_t)ntl * sizeof(int)));
+        RC_HIP(hipMemcpyAsync(h->tl_start.p, ls.data(), (size_t)ntl * sizeof(int), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->tl_cnt.p, lc.data(), (size_t)ntl * sizeof(int), hipMemcpyHostToDevice, s));
+        RC_HIP(h->tacc.ensure((size_t)nt * sizeof(T)));
+        RC_HIP(h->ti_ptr.ensure(((size_t)nt + 1) * sizeof(int)));
+        int *cnt_b = h->tmp_a.as<int>(), *cnt_i = cnt_b + nt + 1;
+        auto build = [&](const EllSrc &src, bool fwd, DevVec &off, DevVec &slices, DevVec &ind, DevVec &val) -> int {
+            launch_tail_count(nt, tp0, h->order.as<int>(), src, cnt_b, cnt_i, s);
+            if (fwd) RC_HIP(h->tp_ptr.ensure(((size_t)nt + 1) * sizeof(int)));
+            int nb = 0, ni = 0;
+            if (fwd) {
+                scan_exclusive(cnt_b, h->tp_ptr.as<int>(), nt, h->tmp_scan.as<int>(), s);
+                RC_HIP(hipMemcpyAsync(&nb, h->tp_ptr.as<int>() + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+            }
+            scan_exclusive(cnt_i, h->ti_ptr.as<int>(), nt, h->tmp_scan.as<int>(), s);
+            RC_HIP(hipMemcpyAsync(&ni, h->ti_ptr.as<int>() + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+            RC_HIP(hipStreamSynchronize(s));
+            if (!fwd) {
+                // a backward dependency always lies in a deeper level, so the
+                // backward tail has no bulk part
+                int64_t tot = 0;
+                std::vector<int> hb((size_t)nt);
+                RC_HIP(hipMemcpy(hb.data(), cnt_b, (size_t)nt * sizeof(int), hipMemcpyDeviceToHost));
+                for (int v : hb) tot += v;
+                if (tot != 0) return fail(h, RC_ERR_STATE, "backward tail row depends on a wider level");
+            }
+            if (fwd) {
+                RC_HIP(h->tp_ind.ensure((size_t)std::max(nb, 1) * sizeof(int)));
+                RC_HIP(h->tp_val.ensure((size_t)std::max(nb, 1) * sizeof(T)));
+            }
+            RC_HIP(h->ti_ind.ensure((size_t)std::max(ni, 1) * sizeof(int)));
+            RC_HIP(h->ti_val.ensure((size_t)std::max(ni, 1) * sizeof(T)));
+            launch_tail_split<T>(nt, tp0, h->order.as<int>(), src, fwd ? h->tp_ptr.as<int>() : cnt_b,
+                                 h->ti_ptr.as<int>(), fwd ? h->tp_ind.as<int>() : nullptr,
+                                 fwd ? h->tp_val.as<T>() : nullptr, h->ti_ind.as<int>(), h->ti_val.as<T>(), s);
+            RC_HIP(slices.ensure((size_t)ntl * sizeof(int)));
+            RC_HIP(off.ensure(((size_t)ntl + 1) * sizeof(int)));
+            int *size = cnt_b;        // reuse: ntl <= nt
+            launch_tail_layout(ntl, h->tl_start.as<int>(), h->tl_cnt.as<int>(), tp0, h->ti_ptr.as<int>(),
+                               slices.as<int>(), size, s);
+            scan_exclusive(size, off.as<int>(), ntl, h->tmp_scan.as<int>(), s);
+            int total = 0;
+            RC_HIP(hipMemcpyAsync(&total, off.as<int>() + ntl, sizeof(int), hipMemcpyDeviceToHost, s));
+            RC_HIP(hipStreamSynchronize(s));
+            RC_HIP(ind.ensure((size_t)std::max(total, 1) * sizeof(int)));
+            RC_HIP(val.ensure((size_t)std::max(total, 1) * sizeof(T)));
+            launch_tail_fill<T>(ntl, h->tl_start.as<int>(), h->tl_cnt.as<int>(), tp0, h->ti_ptr.as<int>(),
+                                h->ti_ind.as<int>(), h->ti_val.as<T>(), off.as<int>(), slices.as<int>(), nt,
+                                ind.as<int>(), val.as<T>(), s);
+            RC_HIP(hipStreamSynchronize(s));
+            return RC_OK;
+        };
+        rc = build(fs, true, h->tf_off, h->tf_slices, h->tf_ind, h->tf_val);
+        if (rc != RC_OK) return rc;
+        rc = build(bs, false, h->tb_off, h->tb_slices, h->tb_ind, h->tb_val);
+        if (rc != RC_OK) return rc;
+    }
 
     // 5. A in position numbering, sliced ELL over 64-position tiles
     const int ntiles = (ns + 63) / 64;
@@ -1309,6 +1402,8 @@ static SolveArgs solve_args(rc_handle_t h)
     a.nchunks = h->nchunks;
     a.chunk_start = h->chunk_start.as<int>();
     a.chunk_cnt = h->chunk_cnt.as<int>();
+    a.cm_f = h->cm_f.as<int4>();
+    a.cm_b = h->cm_b.as<int4>();
     a.lf_off = h->lf_off.as<int>();
     a.lf_width = h->lf_width.as<int>();
     a.lf_ind = h->lf_ind.as<int>();
@@ -1333,8 +1428,25 @@ static SolveArgs solve_args(rc_handle_t h)
     a.sweep_blocks = h->sweep_blocks;
     a.bulk_chunks = h->bulk_chunks;
     a.tail_level = h->tail_level;
+    a.tail_pos0 = h->tail_pos0;
+    a.ntail = h->ntail;
+    a.ntail_levels = h->ntail_levels;
+    a.tail_lds_bytes = h->tail_lds_bytes;
+    a.tl_start = h->tl_start.as<int>();
+    a.tl_cnt = h->tl_cnt.as<int>();
+    a.tf_off = h->tf_off.as<int>();
+    a.tf_slices = h->tf_slices.as<int>();
+    a.tf_ind = h->tf_ind.as<int>();
+    a.tf_val = h->tf_val.p;
+    a.tb_off = h->tb_off.as<int>();
+    a.tb_slices = h->tb_slices.as<int>();
+    a.tb_ind = h->tb_ind.as<int>();
+    a.tb_val = h->tb_val.p;
+    a.tp_ptr = h->tp_ptr.as<int>();
+    a.tp_ind = h->tp_ind.as<int>();
+    a.tp_val = h->tp_val.p;
+    a.tacc = h->tacc.p;
     a.nlevels = h->nlevels;
-    a.tail_cptr = h->tail_cptr.as<int>();
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
     a.sys_of = nullptr;

@@ -7,6 +7,7 @@
 
 #include "../../../include/rchol.h"
 #include "../device/kernels.h"
+#include "philox.h"
 
 namespace rc {
 
@@ -62,9 +63,19 @@ void gnn_graph(const Csc &a, int nvert, GnnGraph &g)
 
 std::vector<int> order_from_scores(const std::vector<float> &scores, uint64_t seed)
 {
+    // One draw from the Plackett-Luce distribution of the scores (the
+    // distribution the network is trained under): sort score + Gumbel noise
+    // in descending order.  Gumbel(0,1) = -log(-log u), u uniform in (0,1).
     const int n = (int)scores.size();
-    std::vector<int> p = random_perm(n, seed);
-    std::stable_sort(p.begin(), p.end(), [&](int x, int y) { return scores[(size_t)x] > scores[(size_t)y]; });
+    std::vector<double> key((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const U4 o = philox_keyed(seed, (uint32_t)i, 0u, kTagGumbel);
+        const double u = ((double)((((uint64_t)o.y << 32) | o.x) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        key[(size_t)i] = (double)scores[(size_t)i] - std::log(-std::log(u));
+    }
+    std::vector<int> p((size_t)n);
+    std::iota(p.begin(), p.end(), 0);
+    std::stable_sort(p.begin(), p.end(), [&](int x, int y) { return key[(size_t)x] > key[(size_t)y]; });
     return p;
 }
 

@@ -22,8 +22,9 @@ struct GnnGraph {
 };
 void gnn_graph(const Csc &a, int nvert, GnnGraph &g);
 
-// Elimination order from scores: descending score, ties broken by a seeded
-// random permutation (so equal scores fall back to a random order).
+// Elimination order from scores: a seeded draw from the Plackett-Luce
+// distribution of the scores (descending score + Gumbel noise), so equal
+// scores give a uniformly random order.
 std::vector<int> order_from_scores(const std::vector<float> &scores, uint64_t seed);
 
 }  // namespace rc

@@ -16,7 +16,7 @@
 namespace rc {
 
 // Counter word 2 separates the consumers of random numbers.
-enum : uint32_t { kTagSample = 0, kTagNormal = 1, kTagPerm = 2, kTagWeight = 3, kTagPoint = 4 };
+enum : uint32_t { kTagSample = 0, kTagNormal = 1, kTagPerm = 2, kTagWeight = 3, kTagPoint = 4, kTagGumbel = 5 };
 
 struct U4 { uint32_t x, y, z, w; };
 

@@ -87,8 +87,20 @@ def test_gpu_forward_backward_match_oracle(rc, wl, dims):
     assert np.linalg.norm(dwg - dwo) <= 1e-2 * np.linalg.norm(dwo)
 
 
+def _gumbel_keys(orc, scores, seed):
+    """score_i - log(-log u_i), u_i from Philox4x32-10 keyed (seed; i, 0, tag 5),
+    53 bits + 1/2 scaled by 2^-53: the seeded Plackett-Luce draw."""
+    keys = np.empty(len(scores))
+    for i, sc in enumerate(scores):
+        o = orc.philox4x32([i, 0, 5, 0], [seed & 0xFFFFFFFF, seed >> 32])
+        bits = ((int(o[1]) << 32) | int(o[0])) >> 11
+        u = (bits + 0.5) / 2.0 ** 53
+        keys[i] = float(sc) - np.log(-np.log(u))
+    return keys
+
+
 @pytest.mark.gpu
-def test_gpu_learned_order_is_a_permutation_and_fallback(rc, wl):
+def test_gpu_learned_order_is_a_permutation_and_fallback(rc, wl, orc):
     print("[gpu] learned order", flush=True)
     n, r, c, v = wl.grid_laplacian((10, 10), "loguniform", seed=2)
     s = rc.Solver(device=0)
@@ -104,7 +116,10 @@ def test_gpu_learned_order_is_a_permutation_and_fallback(rc, wl):
     p = s.permutation()
     assert sorted(p.tolist()) == list(range(n))
     sc = s.gnn_scores(w)
-    assert np.all(np.diff(sc[p]) <= 0)          # descending score
+    keys = _gumbel_keys(orc, sc, 1)
+    assert np.all(np.diff(keys[p]) <= 0)        # descending score + Gumbel(seed)
+    s.order(rc.ORDER_LEARNED, 2)
+    assert not np.array_equal(s.permutation(), p)   # another seed, another draw
 
 
 def test_plackett_luce_gradient_matches_finite_differences():
