@@ -180,6 +180,13 @@ __host__ __device__ inline int tail_group(int cnt)
     int g = kTailThreads / sweep_rows_pow2(cnt);
     return g > 64 ? 64 : (g < 1 ? 1 : g);
 }
+// LDS of the tail workgroup: ntail + 1 values (16-byte padded), then one
+// int4 descriptor per tail level.
+template <typename T>
+__host__ __device__ inline int tail_value_bytes(int ntail)
+{
+    return (int)((((long long)ntail + 1) * (long long)sizeof(T) + 15) / 16 * 16);
+}
 void launch_tail_count(int ntail, int tp0, const int *order, EllSrc src, int *cnt_b, int *cnt_i, hipStream_t s);
 template <typename TD>
 void launch_tail_split(int ntail, int tp0, const int *order, EllSrc src, const int *ptr_b, const int *ptr_i,
@@ -246,6 +253,7 @@ struct SolveArgs {
     void *tacc;
     long long spin_ticks;
     const double *b;
+    int zero;                     // always 0 (keeps some loads per-lane, see k_sweep_bulk)
     // batched systems: system of every position and the per-system alpha
     // used by the forward sweep's r/x update (both null for one system)
     const int *sys_of;

@@ -299,39 +299,69 @@ __device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
 
 // Persistent part: workgroups claim chunks of the wide levels in order
 // (forward) or reverse order (backward); dependencies are awaited per value.
-// Thread 0 holds the claim of the next chunk while the current one is
-// processed, and fetches the chunk's packed descriptor.
+// Claims run ahead of the work: while chunk i is processed, the workgroup
+// already knows chunk i+1 (every thread loads its descriptor in the
+// meantime) and thread 0 has the claim of chunk i+2 in flight, so no round
+// trip is exposed between chunks.  Holding claims ahead cannot deadlock: the
+// smallest unfinished claimed chunk is always some workgroup's current chunk,
+// and its dependencies are all earlier chunks.
 template <typename T, bool kFwd>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const double *p_cur, int update_rx)
 {
-    __shared__ int4 s_meta;
-    __shared__ int s_go;
+    __shared__ int s_claim[2];
+    __shared__ int s_stop;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
     unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
     const int4 *meta = kFwd ? a.cm_f : a.cm_b;
+    const int nb = a.bulk_chunks;
     const int t = threadIdx.x;
-    int next = 0;
-    if (t == 0) next = (int)atomicAdd(ctr, 1u);
-    while (true) {
+    // descriptor of claim c (clamped: a claim past the end is never processed).
+    // The lane term (a.zero is 0) keeps the load a per-lane one, so the
+    // compiler does not move the value to scalar registers (and wait for it)
+    // at the load; it is made uniform only when it is used, a chunk later.
+    auto desc = [&](int c) {
+        const int cc = c < nb ? c : nb - 1;
+        return meta[(kFwd ? cc : nb - 1 - cc) + lane_id() * a.zero];
+    };
+    auto uniform = [](int4 v) {
+        return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                         __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+    };
+    int c_pend = 0;
+    unsigned int st_pend = 0;
+    if (t == 0) {
+        s_claim[0] = (int)atomicAdd(ctr, 1u);
+        s_claim[1] = (int)atomicAdd(ctr, 1u);
+        c_pend = (int)atomicAdd(ctr, 1u);
+        st_pend = ld_relaxed(&sc->status);
+    }
+    __syncthreads();
+    int c_cur = s_claim[0], c_nxt = s_claim[1];
+    int4 m_cur = uniform(desc(c_cur));
+    int par = 0;
+    while (c_cur < nb) {
+        const int4 m_nxt = desc(c_nxt);
+        if (kFwd) fwd_chunk<T>(a, m_cur, t, alpha, p_cur, update_rx);
+        else bwd_chunk<T>(a, m_cur, t);
+        // made uniform before thread 0 issues new loads, so that waiting for
+        // it does not also wait for them
+        const int4 m_next_u = uniform(m_nxt);
         if (t == 0) {
-            const unsigned int st = ld_relaxed(&sc->status);
-            const int c = next;
-            const bool go = c < a.bulk_chunks;
-            int4 m = make_int4(0, 0, 0, 0);
-            if (go) m = meta[kFwd ? c : a.bulk_chunks - 1 - c];
-            if (go) next = (int)atomicAdd(ctr, 1u);
-            s_meta = m;
-            s_go = (go && !(st & kPcgTimeout)) ? 1 : 0;
+            // publish the claim taken a chunk ago, take the next one; both
+            // this atomic and the status load complete during the next chunk
+            s_claim[par] = c_pend;              // slots alternate, so one barrier suffices
+            s_stop = (int)st_pend;
+            c_pend = (int)atomicAdd(ctr, 1u);
+            st_pend = ld_relaxed(&sc->status);
         }
         __syncthreads();
-        const int4 m = s_meta;
-        const int go = s_go;
-        __syncthreads();
-        if (!go) break;
-        if (kFwd) fwd_chunk<T>(a, m, t, alpha, p_cur, update_rx);
-        else bwd_chunk<T>(a, m, t);
+        if (s_stop & kPcgTimeout) break;
+        c_cur = c_nxt;
+        c_nxt = s_claim[par];
+        m_cur = m_next_u;
+        par ^= 1;
     }
 }
 
@@ -414,61 +444,60 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         v[q] = init;
     }
     if (t == 0) v[nt] = (T)0;        // slot of the padding entries
-    __syncthreads();
-    // Software pipeline: the descriptor and this lane's first kTailPref
-    // index/value slices of level s + 1 are loaded while level s is solved.
+    // level descriptors {start, cnt, ELL offset, slices} after the values
     const int ntl = a.ntail_levels;
-    int cur_cnt = 0, cur_start = 0, cur_off = 0, cur_nsl = 0;
-    int cj[kTailPref];
-    T cl[kTailPref];
-    auto fetch = [&](int s, int &cnt, int &start, int &o, int &nsl, int *j, T *l) {
-        const int L = kFwd ? s : ntl - 1 - s;
-        cnt = a.tl_cnt[L];
-        start = a.tl_start[L];
-        o = off[L];
-        nsl = slices[L];
-        const int stride = sweep_rows_pow2(cnt) * tail_group(cnt);
+    int4 *tm = reinterpret_cast<int4 *>(reinterpret_cast<char *>(tail_lds_d) + tail_value_bytes<T>(nt));
+    for (int L = t; L < ntl; L += kTailThreads) tm[L] = make_int4(a.tl_start[L], a.tl_cnt[L], off[L], slices[L]);
+    __syncthreads();
+    // Software pipeline over two register sets (A, B): this lane's first
+    // kTailPref index/value slices of level s + 1 are loaded while level s is
+    // solved; the loop is unrolled by two so that no register copy (and no
+    // wait) sits between the loads and their use.
+    // Loads are issued unconditionally (a lane without a slice reads entry 0
+    // and discards it), so every path has the same number of loads in flight
+    // and the compiler's waits stay exact.
+    auto fetch = [&](int s, int4 &d, int *j, T *l) {
+        s = s < ntl ? s : ntl - 1;
+        d = tm[kFwd ? s : ntl - 1 - s];
+        const int stride = sweep_rows_pow2(d.y) * tail_group(d.y);
 #pragma unroll
         for (int u = 0; u < kTailPref; ++u) {
-            j[u] = nt;
-            l[u] = (T)0;
-            if (t < stride && u < nsl) {
-                j[u] = ind[o + t + u * stride];
-                l[u] = val[o + t + u * stride];
-            }
+            const bool has = t < stride && u < d.w;
+            const int e = has ? d.z + t + u * stride : 0;
+            const int jj = ind[e];
+            const T ll = val[e];
+            j[u] = has ? jj : nt;
+            l[u] = has ? ll : (T)0;
         }
     };
-    if (ntl > 0) fetch(0, cur_cnt, cur_start, cur_off, cur_nsl, cj, cl);
-    for (int s = 0; s < ntl; ++s) {
-        int nx_cnt = 0, nx_start = 0, nx_off = 0, nx_nsl = 0;
-        int nj[kTailPref];
-        T nl[kTailPref];
-        if (s + 1 < ntl) fetch(s + 1, nx_cnt, nx_start, nx_off, nx_nsl, nj, nl);
-        const int G = tail_group(cur_cnt);
-        const int stride = sweep_rows_pow2(cur_cnt) * G;
+    auto solve = [&](const int4 &d, const int *j, const T *l) {
+        const int G = tail_group(d.y);
+        const int stride = sweep_rows_pow2(d.y) * G;
         const int i = t / G, g = t - i * G;
-        const bool lead = t < stride && i < cur_cnt && g == 0;
-        const int q = cur_start - tp0 + i;
+        const bool lead = t < stride && i < d.y && g == 0;
+        const int q = d.x - tp0 + i;
         T acc = (T)0;
 #pragma unroll
-        for (int u = 0; u < kTailPref; ++u) acc = fma_t(-cl[u], v[cj[u]], acc);
+        for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
         if (t < stride) {
-            for (int sl = kTailPref; sl < cur_nsl; ++sl)
-                acc = fma_t(-val[cur_off + t + sl * stride], v[ind[cur_off + t + sl * stride]], acc);
+            for (int sl = kTailPref; sl < d.w; ++sl)
+                acc = fma_t(-val[d.z + t + sl * stride], v[ind[d.z + t + sl * stride]], acc);
         }
         if (lead) acc += v[q];
         acc = group_sum(acc, G);
         if (lead) v[q] = acc;
         lds_barrier();
-        cur_cnt = nx_cnt;
-        cur_start = nx_start;
-        cur_off = nx_off;
-        cur_nsl = nx_nsl;
-#pragma unroll
-        for (int u = 0; u < kTailPref; ++u) {
-            cj[u] = nj[u];
-            cl[u] = nl[u];
-        }
+    };
+    int4 dA = make_int4(0, 0, 0, 0), dB = make_int4(0, 0, 0, 0);
+    int jA[kTailPref], jB[kTailPref];
+    T lA[kTailPref], lB[kTailPref];
+    if (ntl > 0) fetch(0, dA, jA, lA);
+    for (int s = 0; s < ntl; s += 2) {
+        fetch(s + 1, dB, jB, lB);
+        solve(dA, jA, lA);
+        if (s + 1 >= ntl) break;
+        fetch(s + 2, dA, jA, lA);
+        solve(dB, jB, lB);
     }
     for (int q = t; q < nt; q += kTailThreads) {
         const int k = tp0 + q;

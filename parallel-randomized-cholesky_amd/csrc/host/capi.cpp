@@ -975,12 +975,12 @@ static int assemble(rc_handle_t h, double *ms_out)
         RC_HIP(hipMemcpyAsync(hc.data(), cptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
         RC_HIP(hipStreamSynchronize(s));
         if (h->tail_lds_cap < 0) h->tail_lds_cap = tail_lds_limit(h->opt.device);
-        const int64_t cap_rows = (int64_t)h->tail_lds_cap / (int64_t)sizeof(T) - 2;
         int tl = nl;
         int64_t rows = 0;
         while (tl > 0) {
             const int c = hl[(size_t)tl] - hl[(size_t)tl - 1];
-            if (c >= h->tail_max_rows || rows + c > cap_rows) break;
+            const int64_t bytes = ((rows + c + 1) * (int64_t)sizeof(T) + 15) / 16 * 16 + 16 * (int64_t)(nl - tl + 1);
+            if (c >= h->tail_max_rows || bytes > (int64_t)h->tail_lds_cap) break;
             rows += c;
             --tl;
         }
@@ -1044,7 +1044,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     h->ntail_levels = nl - h->tail_level;
     h->tail_pos0 = nl > 0 ? hl[(size_t)h->tail_level] : ns;
     h->ntail = ns - h->tail_pos0;
-    h->tail_lds_bytes = (int)((((size_t)h->ntail + 1) * sizeof(T) + 15) / 16 * 16);
+    h->tail_lds_bytes = tail_value_bytes<T>(h->ntail) + 16 * h->ntail_levels;
     if (h->ntail > 0) {
         const int ntl = h->ntail_levels, tp0 = h->tail_pos0, nt = h->ntail;
         std::vector<int> ls((size_t)ntl), lc((size_t)ntl);
@@ -1449,6 +1449,7 @@ static SolveArgs solve_args(rc_handle_t h)
     a.nlevels = h->nlevels;
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
+    a.zero = 0;
     a.sys_of = nullptr;
     a.alpha_sys = nullptr;
     if (h->nsys > 0) {
