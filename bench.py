@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-max-n", type=int, default=2_200_000)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the per-structure sweep launch tuning done once before the timed steps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md); run with --gpus 1",
@@ -234,6 +236,14 @@ def main():
         log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f}s iters={sst['iterations']} "
             f"factor={fst['factor_ms']:.1f}ms assemble={fst['assemble_ms']:.1f}ms pcg={sst['pcg_ms']:.1f}ms")
 
+    tune = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tune = s.autotune(reps=3)
+        tune["autotune_s"] = time.perf_counter() - t
+        log(f"[bench] autotune: tail limit {tune['tail_max_rows']} grid {tune['sweep_blocks']} sweeps "
+            f"{tune['sweeps_ms_before']:.3f} -> {tune['sweeps_ms_after']:.3f} ms ({tune['autotune_s']:.2f}s)")
+
     device_sync()
     times, fms, ams, pms, iters = [], [], [], [], []
     t_all = time.perf_counter()
@@ -291,6 +301,7 @@ def main():
             "iterations": iters, "rel_res_true": sst["rel_res_true"], "nnz_factor": fst["nnz_factor"],
             "fills": fst["fills"], "nlevels": fst["nlevels"], "max_degree": fst["max_degree"],
             "kernels": kern,
+            "autotune": tune,
         },
     }
     if not args.no_cpu:

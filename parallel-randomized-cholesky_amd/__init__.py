@@ -39,8 +39,17 @@ EXPORTS = (
     "rc_solve", "rc_make_rhs", "rc_set_reference_solution", "rc_set_rhs", "rc_solve_resident",
     "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
     "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
-    "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats",
+    "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats", "rc_autotune",
 )
+
+
+class TuneStats(ctypes.Structure):
+    _fields_ = [("tail_max_rows", ctypes.c_int32), ("sweep_blocks", ctypes.c_int32),
+                ("configs_tried", ctypes.c_int32), ("sweeps_ms_before", ctypes.c_double),
+                ("sweeps_ms_after", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class RcholError(RuntimeError):
@@ -135,6 +144,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_get_batch.argtypes = [vp, P(ctypes.c_int), vp]
     lib.rc_solve_batch.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.rc_get_batch_stats.argtypes = [vp, vp]
+    lib.rc_autotune.argtypes = [vp, ctypes.c_int, P(TuneStats)]
     _lib = lib
     return lib
 
@@ -352,6 +362,12 @@ class Solver:
         x = np.empty(n, dtype=np.float64)
         self._check(self.lib.rc_get_solution(self.h, _ptr(x)))
         return x
+
+    def autotune(self, reps=5):
+        """Pick the sweep launch configuration for the factored structure."""
+        st = TuneStats()
+        self._check(self.lib.rc_autotune(self.h, int(reps), ctypes.byref(st)))
+        return st.as_dict()
 
     def time_kernel(self, kernel, reps=10):
         ms, by = ctypes.c_double(), ctypes.c_double()

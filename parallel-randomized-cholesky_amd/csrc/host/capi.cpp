@@ -118,6 +118,7 @@ struct rc_handle_s {
     DevVec calib;                   // copy-kernel calibration buffer (rc_time_kernel)
     int reduce_blocks = 1024;
     int sweep_blocks = 1024;
+    int sweep_blocks_max = 1024;    // occupancy limit of the persistent sweep grid
     bool rhs_set = false;
     std::vector<double> xref;
     hipGraphExec_t graph = nullptr;
@@ -244,6 +245,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
     h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
     h->sweep_blocks = sweep_occupancy_blocks(o.device);
+    h->sweep_blocks_max = h->sweep_blocks;
     {
         const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
         h->spatial_levels = lo && std::string(lo) == "spatial";
@@ -2056,6 +2058,66 @@ int rc_solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
     if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and rc_set_rhs must precede");
     RC_HIP(hipSetDevice(h->opt.device));
     return solve_resident(h, tol, st);
+}
+
+// Autotuner: per factored structure, time both sweeps (forward + backward,
+// hipEvents, `reps` launches each) for every candidate (tail width limit x
+// sweep grid) and keep the fastest.  Changing the tail limit re-runs only the
+// assembly (the elimination's output stays on the device).
+int rc_autotune(rc_handle_t h, int reps, rc_tune_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and rc_set_rhs must precede rc_autotune");
+    if (reps <= 0) reps = 5;
+    RC_HIP(hipSetDevice(h->opt.device));
+    const int tails[] = {0, 64, 128, 256, 512, 1025};
+    const int grids[] = {h->sweep_blocks_max, h->sweep_blocks_max / 2, h->sweep_blocks_max / 4};
+    auto reassemble = [&](int tail) -> int {
+        h->tail_max_rows = tail;
+        double ms = 0.0;
+        int rc = (h->prec == 32) ? assemble<float>(h, &ms) : assemble<double>(h, &ms);
+        invalidate_graph(h);
+        return rc;
+    };
+    auto sweep_ms = [&](double *out) -> int {
+        double f = 0.0, b = 0.0;
+        int rc = rc_time_kernel(h, RC_KERNEL_TRSV_FWD, reps, &f, nullptr);
+        if (rc == RC_OK) rc = rc_time_kernel(h, RC_KERNEL_TRSV_BWD, reps, &b, nullptr);
+        *out = f + b;
+        return rc;
+    };
+    const int tail0 = h->tail_max_rows, grid0 = h->sweep_blocks;
+    double base = 0.0;
+    int rc = sweep_ms(&base);
+    if (rc != RC_OK) return rc;
+    double best = base;
+    int best_tail = tail0, best_grid = grid0, tried = 1;
+    for (int tail : tails) {
+        rc = reassemble(tail);
+        if (rc != RC_OK) return rc;
+        for (int g : grids) {
+            if (g < 1) continue;
+            h->sweep_blocks = g;
+            double ms = 0.0;
+            rc = sweep_ms(&ms);
+            if (rc != RC_OK) return rc;
+            ++tried;
+            if (ms < best) { best = ms; best_tail = tail; best_grid = g; }
+        }
+    }
+    h->sweep_blocks = best_grid;
+    rc = reassemble(best_tail);
+    if (rc != RC_OK) return rc;
+    h->fstats.nlevels = h->nlevels;
+    if (st) {
+        st->tail_max_rows = best_tail;
+        st->sweep_blocks = best_grid;
+        st->configs_tried = tried;
+        st->sweeps_ms_before = base;
+        st->sweeps_ms_after = best;
+    }
+    return RC_OK;
 }
 
 int rc_get_solution(rc_handle_t h, double *x)

@@ -189,6 +189,41 @@ def test_resident_solve_and_kernel_timing(rc, wl):
         assert ms > 0 and by > 0
 
 
+@pytest.mark.parametrize("tail", ["0", "64", "1024"])
+def test_tail_width_limits_agree_with_oracle(rc, orc, wl, tail, monkeypatch):
+    """Every tail width limit (no tail; narrow; up to 1024-row levels in the
+    LDS workgroup) solves the same system to the oracle's answer."""
+    say(f"[gpu] tail limit {tail}")
+    monkeypatch.setenv("RCHOL_TAIL_ROWS", tail)
+    s, data, kind = _solver(rc, wl, "grid3d_20_log", 64)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=64)
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    assert np.linalg.norm(x[perm] - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+def test_autotune_keeps_the_solution(rc, orc, wl):
+    say("[gpu] autotune")
+    s, data, kind = _solver(rc, wl, "grid3d_20_log", 32)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    s.set_rhs(b)
+    t = s.autotune(reps=2)
+    assert t["configs_tried"] > 1 and t["sweeps_ms_after"] <= t["sweeps_ms_before"]
+    st = s.solve_resident(1e-6)
+    x = s.get_solution()
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=32)
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    assert np.linalg.norm(x[perm] - xo) <= 1e-4 * np.linalg.norm(xo)
+    s.factor()                                   # a new factor keeps the tuned configuration
+    x2, st2 = s.solve(b, tol=1e-6)
+    assert abs(st2["iterations"] - so["iterations"]) <= 1
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("k", [1, 5, 64])
 def test_multi_rhs_matches_oracle(rc, orc, wl, k, prec):
