@@ -5,25 +5,26 @@
 // r'z / p'Ap; x += alpha p; r -= alpha Ap; z = M r; beta = r'z_new / r'z;
 // p = z + beta p; stop on sqrt(r'z / r0'z0) <= tol or k = maxit).
 //
-// One iteration is four kernels with device-resident scalars and no host
-// synchronisation, so a run of iterations is captured in one hipGraph:
-//   K1 spmv   p_new = z + beta p_old (fused into the SpMV gather), Ap = A p_new,
-//             block partials of p'Ap, last block computes alpha;
+// One iteration is a handful of launches with device-resident scalars and no
+// host synchronisation, so a run of iterations is captured in one hipGraph:
+//   K1 p_new = z + beta p_old (streaming), then the SpMV Ap = A p_new with
+//      block partials of p'Ap; the last block computes alpha;
 //   K2 fwd    r -= alpha Ap and x += alpha p on the row being solved, then the
-//             forward sweep y = L^-1 r over level-ordered chunks;
-//   K3 bwd    z = L^-T (D+ y) over chunks in reverse level order;
+//             forward sweep y = L^-1 r: persistent chunks over the wide levels,
+//             then the narrow trailing levels in one LDS-resident workgroup;
+//   K3 bwd    z = L^-T (D+ y): the trailing levels first, then the wide ones;
 //   K4 dot    r'z block partials, last block computes beta and the stop test.
-// All vectors live in the level-order position numbering, so a 64-row chunk of
-// the sweeps (and a 64-row tile of the SpMV) reads and writes its own rows
-// with coalesced accesses; L, L^T and A are stored as sliced ELL (entry t of
-// slice row l at off + 64 t + l), so index/value loads are coalesced too.
-// Padding entries point at the dummy slot n, which always holds zero.
-// The sweeps are persistent: a wave claims the next 64-row chunk of the level
-// order; a row issues its dependency loads eight at a time with plain (cached)
-// loads and, only for a value still holding the sentinel, polls it with
-// agent-scope loads.  Values are written once per sweep with agent-scope
-// stores, so no fences are needed; each consumer resets the vector it drains
-// back to the sentinel.  All reductions use a fixed grid and a fixed order.
+// All vectors live in the level-order position numbering, so a chunk of the
+// sweeps (and a 64-row tile of the SpMV) reads and writes its own rows with
+// coalesced accesses; L, L^T and A are stored as sliced ELL, so index/value
+// loads are coalesced too.  Padding entries point at the dummy slot n, which
+// always holds zero.  In the persistent sweeps a workgroup claims the next
+// chunk of the level order; a row issues its dependency loads eight at a time
+// with plain (cached) loads and, only for a value still holding the sentinel,
+// polls it with agent-scope loads.  Values are written once per sweep with
+// agent-scope stores, so no fences are needed; each consumer resets the vector
+// it drains back to the sentinel.  All reductions use a fixed grid and a fixed
+// order.
 #include "common.cuh"
 #include "kernels.h"
 
@@ -132,20 +133,32 @@ __device__ __forceinline__ T group_sum(T v, int G)
 }
 
 // ---------------------------------------------------------------- K1
+// p_new = z + beta p_old (streaming), then Ap = A p_new with block partials of
+// p'Ap; the last block computes alpha.  (Forming p_new inside the SpMV gather
+// would recompute it once per matrix entry.)
 template <typename T>
-__global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p_old, double *p_new, int first)
+__global__ void __launch_bounds__(kRedBlock) k_pupdate(SolveArgs a, const double *p_old, double *p_new)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const double beta = sc->beta;
+    const T *z = (const T *)a.z;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+        p_new[i] = fma_t(beta, p_old[i], (double)z[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p)
 {
     __shared__ double lds[kRedBlock / kWave];
     __shared__ int flag;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
-    const double beta = first ? 0.0 : sc->beta;
-    const T *z = (const T *)a.z;
     const int lane = lane_id();
     double acc_dot = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
-        const int tile = p / kWave;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ntiles * kWave; q += gridDim.x * blockDim.x) {
+        const int tile = q / kWave;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
@@ -157,19 +170,14 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
 #pragma unroll
             for (int u = 0; u < 4; ++u) { j[u] = ind[(t + u) * kWave]; v[u] = val[(t + u) * kWave]; }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = beta * p_old[j[u]] + (double)z[j[u]];
+            for (int u = 0; u < 4; ++u) pj[u] = p[j[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
         }
-        for (; t < w; ++t) {
-            int j = ind[t * kWave];
-            acc = fma_t(val[t * kWave], beta * p_old[j] + (double)z[j], acc);
-        }
-        if (p < a.n) {
-            double pi = beta * p_old[p] + (double)z[p];
-            p_new[p] = pi;
-            a.ap[p] = acc;
-            acc_dot += pi * acc;
+        for (; t < w; ++t) acc = fma_t(val[t * kWave], p[ind[t * kWave]], acc);
+        if (q < a.n) {
+            a.ap[q] = acc;
+            acc_dot += p[q] * acc;
         }
     }
     double bs = block_sum(acc_dot, lds);
@@ -887,7 +895,8 @@ void launch_pcg_iteration(const SolveArgs &a, int parity, hipStream_t s)
 {
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
-    hipLaunchKernelGGL(k_spmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_old, p_new, 0);
+    hipLaunchKernelGGL(k_pupdate<T>, dim3(2048), dim3(kRedBlock), 0, s, a, (const double *)p_old, p_new);
+    hipLaunchKernelGGL(k_spmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_new);
     launch_fwd<T>(a, (const double *)p_new, 1, s);
     launch_bwd<T>(a, s);
     hipLaunchKernelGGL(k_dot<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, 0);
