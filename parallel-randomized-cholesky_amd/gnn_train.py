@@ -6,9 +6,14 @@ from the Plackett-Luce distribution of the scores (sort of score + Gumbel
 noise, descending = eliminated first).  Each sampled ordering is run through
 the method itself on the GPU -- randomized factor + PCG to the tolerance --
 and its cost
-    cost = log(PCG iterations) + 0.5 log(levels of the triangular solves)
-           + 0.5 log(factor nnz)
-is the training signal (REINFORCE with the batch mean as baseline).  The
+    cost = log(PCG iterations) + log(levels of the triangular solves)
+           + 0.25 log(factor nnz)
+is the training signal (REINFORCE with the batch mean as baseline).  On the
+GPU the elimination takes one round per level and every PCG iteration walks
+all levels twice, so iterations x levels is the proxy for the time.  The
+network is first fitted (mean squared error, same backward kernels) to a
+simple prior -- eliminate vertices of small weighted degree first, scores
+-2 (log weighted degree - mean) -- and then trained on the method's costs.  The
 gradient of log p(ordering) with respect to the scores has a closed form
 (prefix sums over the ordering); the gradient with respect to the weights comes
 from the library's backward kernels (rc_gnn_gradient).  Adam runs on the host
@@ -78,8 +83,21 @@ def pl_logp_grad(scores: np.ndarray, perm: np.ndarray) -> np.ndarray:
 
 
 def cost_of(fs: dict, ss: dict) -> float:
-    return float(np.log(max(ss["iterations"], 1)) + 0.5 * np.log(max(fs["nlevels"], 1))
-                 + 0.5 * np.log(max(fs["nnz_factor"], 1)))
+    return float(np.log(max(ss["iterations"], 1)) + np.log(max(fs["nlevels"], 1))
+                 + 0.25 * np.log(max(fs["nnz_factor"], 1)))
+
+
+def prior_scores(n, rows, cols, vals, kappa=2.0):
+    """-kappa (log weighted degree - mean over vertices with edges)."""
+    rows = np.asarray(rows); cols = np.asarray(cols); vals = np.abs(np.asarray(vals, dtype=np.float64))
+    off = rows != cols
+    wdeg = np.bincount(rows[off], weights=vals[off], minlength=n)
+    t = np.zeros(n)
+    has = wdeg > 0
+    if has.any():
+        lw = np.log(wdeg[has])
+        t[has] = -kappa * (lw - lw.mean())
+    return t
 
 
 def training_graphs(wl, small: bool = False):
@@ -96,15 +114,19 @@ def training_graphs(wl, small: bool = False):
 
 
 def evaluate(rc, w, graph, seed=1, precision=64, tol=1e-6):
-    """Cost of the learned ordering and of the nnz-sort fallback on one graph."""
+    """Cost of the learned ordering, of the prior it starts from and of the
+    nnz-sort fallback on one graph."""
     n, r, c, v = graph
     out = {}
-    for name in ("nnz-sort", "learned"):
+    for name in ("nnz-sort", "prior", "learned"):
         s = rc.Solver(device=0, seed=seed, precision=precision)
         s.set_triplets(n, r, c, v)
         if name == "learned":
             s.set_gnn_weights(w)
             s.order(rc.ORDER_LEARNED, seed)
+        elif name == "prior":
+            g = np.random.default_rng(seed).gumbel(size=n)
+            s.set_permutation(np.argsort(-(prior_scores(n, r, c, v) + g), kind="stable").astype(np.int32))
         else:
             s.order(rc.ORDER_NNZ_SORT, seed)
         fs = s.factor()
@@ -116,7 +138,31 @@ def evaluate(rc, w, graph, seed=1, precision=64, tol=1e-6):
     return out
 
 
-def train(steps=300, samples=8, lr=3e-3, seed=0, out=WEIGHTS, small=False, log=print):
+def adam_step(w, dw, m, v2, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    m = b1 * m + (1 - b1) * dw
+    v2 = b2 * v2 + (1 - b2) * dw * dw
+    mh = m / (1 - b1 ** (step + 1))
+    vh = v2 / (1 - b2 ** (step + 1))
+    w = (w.astype(np.float64) - lr * mh / (np.sqrt(vh) + eps)).astype(np.float32)
+    return w, m, v2
+
+
+def pretrain(rc, w, solvers, steps=200, lr=1e-2, log=print):
+    """Fit the scores to the prior by mean squared error."""
+    m = np.zeros_like(w, dtype=np.float64)
+    v2 = np.zeros_like(w, dtype=np.float64)
+    for step in range(steps):
+        name, n, s, b, target = solvers[step % len(solvers)]
+        sc = s.gnn_scores(w).astype(np.float64)
+        ds = 2.0 * (sc - target) / n
+        dw = s.gnn_gradient(w, ds.astype(np.float32)).astype(np.float64)
+        w, m, v2 = adam_step(w, dw, m, v2, step, lr)
+        if step % 20 == 0 or step == steps - 1:
+            log(f"[gnn] prior fit step {step} {name}: mse {np.mean((sc - target) ** 2):.4f}")
+    return w
+
+
+def train(steps=300, samples=8, lr=3e-3, seed=0, out=WEIGHTS, small=False, log=print, prior_steps=200):
     rc = _pkg()
     import importlib
     wl = importlib.import_module("rchol_amd.workloads")
@@ -124,19 +170,20 @@ def train(steps=300, samples=8, lr=3e-3, seed=0, out=WEIGHTS, small=False, log=p
     w = init_weights(nparam, seed)
     m = np.zeros_like(w, dtype=np.float64)
     v2 = np.zeros_like(w, dtype=np.float64)
-    b1, b2, eps = 0.9, 0.999, 1e-8
     graphs = training_graphs(wl, small)
     solvers = []
     for name, (n, r, c, v) in graphs:
         s = rc.Solver(device=0, seed=1)
         s.set_triplets(n, r, c, v)
         b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
-        solvers.append((name, n, s, b))
+        solvers.append((name, n, s, b, prior_scores(n, r, c, v)))
+    if prior_steps > 0:
+        w = pretrain(rc, w, solvers, prior_steps, log=log)
     rng = np.random.default_rng(seed + 1)
     history = []
     t0 = time.time()
     for step in range(steps):
-        name, n, s, b = solvers[step % len(solvers)]
+        name, n, s, b, _ = solvers[step % len(solvers)]
         scores = s.gnn_scores(w).astype(np.float64)
         costs, grads = [], []
         for k in range(samples):
@@ -155,16 +202,12 @@ def train(steps=300, samples=8, lr=3e-3, seed=0, out=WEIGHTS, small=False, log=p
         ds /= samples * n
         # gradient ascent on E[adv * log p]  ==  descent on -that
         dw = -s.gnn_gradient(w, ds.astype(np.float32)).astype(np.float64)
-        m = b1 * m + (1 - b1) * dw
-        v2 = b2 * v2 + (1 - b2) * dw * dw
-        mh = m / (1 - b1 ** (step + 1))
-        vh = v2 / (1 - b2 ** (step + 1))
-        w = (w.astype(np.float64) - lr * mh / (np.sqrt(vh) + eps)).astype(np.float32)
+        w, m, v2 = adam_step(w, dw, m, v2, step, lr)
         history.append({"step": step, "graph": name, "cost_mean": float(costs.mean()), "cost_min": float(costs.min())})
         if step % 10 == 0 or step == steps - 1:
             log(f"[gnn] step {step} {name}: cost mean {costs.mean():.4f} min {costs.min():.4f} "
                 f"({time.time() - t0:.1f}s)")
-    for _, _, s, _ in solvers:
+    for _, _, s, _, _ in solvers:
         s.close()
     w.tofile(out)
     return w, history
@@ -178,10 +221,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--small", action="store_true")
+    ap.add_argument("--prior-steps", type=int, default=200)
     ap.add_argument("--eval", default="", help="comma list of eval graphs: grid3d:N or grid2d:N")
     args = ap.parse_args()
     w, hist = train(args.steps, args.samples, args.lr, args.seed, args.out, args.small,
-                    log=lambda *a: print(*a, flush=True))
+                    log=lambda *a: print(*a, flush=True), prior_steps=args.prior_steps)
     rc = _pkg()
     import importlib
     wl = importlib.import_module("rchol_amd.workloads")
