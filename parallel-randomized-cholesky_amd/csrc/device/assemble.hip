@@ -188,6 +188,21 @@ __global__ void k_level_keys_sys(int ns, const int *level, const int *lptr, int 
     }
 }
 
+// keys[q] = level << 3 | class of row order[q]; class 0 = widest (log2 of the
+// wider of its L and L^T rows >= 7), 7 = empty rows.
+__global__ void k_level_bucket_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                                    uint32_t *keys)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ns; q += gridDim.x * blockDim.x) {
+        const int row = order[q];
+        const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
+        const int w = a > b ? a : b;
+        const int lg = 31 - __clz(w + 1);          // floor(log2(w + 1))
+        const int cls = 7 - (lg < 7 ? lg : 7);
+        keys[q] = ((uint32_t)level[row] << 3) | (uint32_t)cls;
+    }
+}
+
 __global__ void k_inverse_perm(int n, const int *order, int *pos_of)
 {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) pos_of[order[p]] = p;
@@ -554,6 +569,11 @@ void launch_level_keys_sys(int ns, const int *level, const int *lptr, int len_bi
                            uint32_t *keys, hipStream_t s)
 {
     hipLaunchKernelGGL(k_level_keys_sys, dim3(2048), dim3(256), 0, s, ns, level, lptr, len_bits, sys, sys_bits, keys);
+}
+void launch_level_bucket_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                              uint32_t *keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_bucket_keys, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, keys);
 }
 void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s)
 {

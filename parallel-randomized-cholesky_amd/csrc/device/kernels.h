@@ -99,6 +99,9 @@ void launch_row_expand(int ns, const int *ptr, int *rowk, hipStream_t s);
 void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s);
 void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s);
 void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s);
+// keys of rows order[q]: level << 3 | width class (see assemble.hip)
+void launch_level_bucket_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                              uint32_t *keys, hipStream_t s);
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s);
 // Rows per sweep chunk of a level whose widest row (of L and of L^T) has
 // `width` entries: G = next power of two >= width / 8 lanes per row (4..64),

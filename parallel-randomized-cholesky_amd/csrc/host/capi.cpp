@@ -104,7 +104,7 @@ struct rc_handle_s {
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
-    bool spatial_levels = false;
+    int level_order = 0;            // 0 length, 1 spatial, 2 spatial within length buckets
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
     int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
@@ -248,7 +248,8 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     h->sweep_blocks_max = h->sweep_blocks;
     {
         const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
-        h->spatial_levels = lo && std::string(lo) == "spatial";
+        if (lo && std::string(lo) == "spatial") h->level_order = 1;
+        if (lo && std::string(lo) == "bucket") h->level_order = 2;
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
         if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
     }
@@ -926,7 +927,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     // batched systems: (level, system, length) keys keep a system's rows of a
     // level contiguous, so the segmented reductions read runs of positions
     const int sys_bits = (h->nsys > 1 && lvl_bits + bit_width(h->nsys - 1) <= 32) ? bit_width(h->nsys - 1) : 0;
-    const int len_bits = std::max(0, std::min(16, 32 - lvl_bits - sys_bits));
+    int len_bits = std::max(0, std::min(16, 32 - lvl_bits - sys_bits));
     RC_HIP(h->order.ensure(nsz * sizeof(int)));
     RC_HIP(h->pos_of.ensure(nsz * sizeof(int)));
     if (h->nsys > 0) {
@@ -935,7 +936,17 @@ static int assemble(rc_handle_t h, double *ms_out)
         launch_iota(h->order.as<int>(), ns, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + sys_bits + len_bits,
                          h->tmp_b.as<int>(), s);
-    } else if (h->spatial_levels) {
+    } else if (h->level_order == 2 && lvl_bits + 3 <= 32) {
+        // rows of a level in classes of similar width (log2 of the wider of
+        // the L and L^T rows, widest first), each class in the loaded (user)
+        // order: chunks stay near-uniform in width and neighbouring rows of a
+        // chunk are neighbours in the input numbering
+        launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        launch_level_bucket_keys(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(),
+                                 h->lt_ptr.as<int>(), k0, s);
+        radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + 3, h->tmp_b.as<int>(), s);
+        len_bits = 3;
+    } else if (h->level_order == 1) {
         // rows of a level kept in the loaded (user) order: neighbouring rows of
         // a chunk are neighbours in the input numbering, so gathers share lines
         launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index

@@ -205,6 +205,24 @@ def test_tail_width_limits_agree_with_oracle(rc, orc, wl, tail, monkeypatch):
     assert np.linalg.norm(x[perm] - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("order", ["spatial", "bucket"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_level_orders_agree_with_oracle(rc, orc, wl, order, prec, monkeypatch):
+    """The row order inside a level (input order; width classes in input
+    order) changes only the layout: same answer as the oracle."""
+    say(f"[gpu] level order {order} prec={prec}")
+    monkeypatch.setenv("RCHOL_LEVEL_ORDER", order)
+    s, data, kind = _solver(rc, wl, "rgg_3000", prec)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    tol = 1e-7 if prec == 64 else 1e-4
+    assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
+
+
 def test_autotune_keeps_the_solution(rc, orc, wl):
     say("[gpu] autotune")
     s, data, kind = _solver(rc, wl, "grid3d_20_log", 32)
