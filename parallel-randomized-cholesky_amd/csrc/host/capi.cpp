@@ -104,7 +104,7 @@ struct rc_handle_s {
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
-    int level_order = 0;            // 0 length, 1 spatial, 2 spatial within length buckets
+    int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
     int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
@@ -248,6 +248,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     h->sweep_blocks_max = h->sweep_blocks;
     {
         const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
+        if (lo && std::string(lo) == "length") h->level_order = 0;
         if (lo && std::string(lo) == "spatial") h->level_order = 1;
         if (lo && std::string(lo) == "bucket") h->level_order = 2;
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)

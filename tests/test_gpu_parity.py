@@ -205,11 +205,12 @@ def test_tail_width_limits_agree_with_oracle(rc, orc, wl, tail, monkeypatch):
     assert np.linalg.norm(x[perm] - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("order", ["spatial", "bucket"])
+@pytest.mark.parametrize("order", ["length", "bucket"])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_level_orders_agree_with_oracle(rc, orc, wl, order, prec, monkeypatch):
-    """The row order inside a level (input order; width classes in input
-    order) changes only the layout: same answer as the oracle."""
+    """The row order inside a level (by width; width classes in input order;
+    the default is input order) changes only the layout: same answer as the
+    oracle."""
     say(f"[gpu] level order {order} prec={prec}")
     monkeypatch.setenv("RCHOL_LEVEL_ORDER", order)
     s, data, kind = _solver(rc, wl, "rgg_3000", prec)
