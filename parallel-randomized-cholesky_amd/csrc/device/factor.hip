@@ -23,6 +23,9 @@
 //     (new fills with it as the larger end) - (retired multiplicity); the
 //     column whose add reaches zero appends the neighbour to the next round's
 //     list with one wave-aggregated atomic;
+//   * a column's pending fill records hang off kFillLists linked lists
+//     (record index mod kFillLists), so its gather walks them in lockstep,
+//     one lane per list, instead of chasing one long chain;
 //   * fill records and factor entries are carved from per-wave chunks of two
 //     global pools, so pool counters see one atomic per few thousand entries;
 //   * pool or scratch exhaustion raises a status bit, later rounds drain, and
@@ -169,18 +172,29 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, Cursor &
         b.sw[t] = a.lo_w[lo0 + t];
         b.smult[t] = 1;
     }
-    if (lane == 0) {
-        int f = a.head[k];
+    // pending fills: kFillLists lists per column, walked in lockstep (one
+    // lane per list); the order of the gathered entries does not matter, the
+    // (row, weight) sort below makes it canonical
+    {
+        int f = lane < kFillLists ? a.head[(size_t)k * kFillLists + lane] : -1;
         int t = 0;
-        while (f >= 0 && t < nf_in) {
-            FillRec rec = a.pool[f];
-            b.srow[norig + t] = rec.row;
-            b.sw[norig + t] = rec.w;
-            b.smult[norig + t] = 1;
-            f = rec.next;
-            ++t;
+        while (true) {
+            const bool act = f >= 0;
+            const uint64_t mask = ballot(act);
+            if (!mask) break;
+            if (act) {
+                const FillRec rec = a.pool[f];
+                const int pos = t + lanes_below(mask);
+                if (pos < nf_in) {
+                    b.srow[norig + pos] = rec.row;
+                    b.sw[norig + pos] = rec.w;
+                    b.smult[norig + pos] = 1;
+                }
+                f = rec.next;
+            }
+            t += __popcll(mask);
         }
-        if (t != nf_in || f >= 0) raise_status(a.ctr, kStatusInconsistent);
+        if (lane == 0 && t != nf_in) raise_status(a.ctr, kStatusInconsistent);
     }
     __syncthreads();
 
@@ -258,7 +272,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, Cursor &
         int cc = ri > rj ? rj : ri;
         double fw = (b.sw[i] * rest) / S;
         int rec = (int)(fbase + i);
-        int old = atomicExch(&a.head[cc], rec);
+        int old = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
         FillRec fr;
         fr.row = rr;
         fr.next = old;
@@ -343,9 +357,11 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
 __global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, int *nfill, int *level, int *out_cnt,
                               long long *out_start, double *diag)
 {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)n * kFillLists;
+         i += (size_t)gridDim.x * blockDim.x)
+        head[i] = -1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         dep[i] = dep_init[i];
-        head[i] = -1;
         nfill[i] = 0;
         level[i] = 0;
         out_cnt[i] = 0;

@@ -32,12 +32,17 @@ enum : unsigned long long {
     kStatusInconsistent = 16,
 };
 
+// Pending fills of a column are kept in kFillLists singly linked lists
+// (head[k * kFillLists + h]; a record goes to list (record index mod
+// kFillLists)), so a column's gather walks them in parallel.
+constexpr int kFillLists = 16;
+
 struct FactorArgs {
     int n;
     const int *lo_ptr;      // n+1, strict-lower part of the Laplacian (CSC)
     const int *lo_row;
     const double *lo_w;     // |A[r,k]|
-    int *dep, *head, *nfill, *level;
+    int *dep, *head, *nfill, *level;   // head: n * kFillLists
     int *list0, *list1;     // ready lists of even / odd rounds
     unsigned int *round_count;   // [round] = columns eliminated in that round
     long long *wave_state;  // per workgroup: fill cursor, fill end, out cursor, out end

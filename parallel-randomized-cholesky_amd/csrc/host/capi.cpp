@@ -715,7 +715,7 @@ static int alloc_factor_work(rc_handle_t h)
 {
     const size_t nf = (size_t)h->nf;
     RC_HIP(h->dep.ensure(nf * sizeof(int)));
-    RC_HIP(h->head.ensure(nf * sizeof(int)));
+    RC_HIP(h->head.ensure(nf * kFillLists * sizeof(int)));
     RC_HIP(h->nfill.ensure(nf * sizeof(int)));
     RC_HIP(h->level.ensure(nf * sizeof(int)));
     RC_HIP(h->list0.ensure(nf * sizeof(int)));
