@@ -37,7 +37,7 @@ namespace rc {
 
 namespace {
 
-constexpr int kLdsCap = 128;                                   // entries per wave in LDS
+constexpr int kLdsCap = 128;                                   // entries per wave in LDS (round kernel)
 constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc
 constexpr long long kFillChunk = 2048;
 constexpr long long kOutChunk = 2048;
@@ -128,8 +128,8 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
 
 // Eliminates column k; newly ready neighbours are appended to `next`.
 // Returns false if the pools or the scratch are exhausted (status set).
-__device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, Cursor &fcur, Cursor &ocur, int *next,
-                                 unsigned int *next_count)
+__device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, int lds_cap, Cursor &fcur, Cursor &ocur,
+                                 int *next, unsigned int *next_count, unsigned int *defer_count)
 {
     const int lane = lane_id();
     const int last = a.n - 1;
@@ -151,9 +151,14 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, Cursor &
     const int P = next_pow2(m);
     char *base;
     int cap;
-    if (P <= kLdsCap) {
+    if (P > lds_cap && defer_count) {
+        // too big for this kernel's LDS: the round's large-LDS kernel takes it
+        if (lane == 0) a.big_list[atomicAdd(defer_count, 1u)] = k;
+        return true;
+    }
+    if (P <= lds_cap) {
         base = lds;
-        cap = kLdsCap;
+        cap = lds_cap;
     } else if (P <= a.scratch_cap) {
         base = a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry;
         cap = a.scratch_cap;
@@ -343,9 +348,33 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
     bool ok = true;
     for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
         int col = cur[i];
-        if (!eliminate_column(a, col, lds, fcur, ocur, next, &a.round_count[round + 1])) { ok = false; break; }
+        if (!eliminate_column(a, col, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
+                              &a.big_count[round])) { ok = false; break; }
     }
     (void)ok;
+    if (lane_id() == 0) {
+        ws[0] = fcur.cur; ws[1] = fcur.end;
+        ws[2] = ocur.cur; ws[3] = ocur.end;
+    }
+}
+
+// The round's columns deferred by k_factor_round (more than kLdsCap entries):
+// one wave per column with kBigCap entries of LDS (columns above that use the
+// per-workgroup global scratch).  Runs after k_factor_round of the same round.
+__global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int round, int *next)
+{
+    extern __shared__ __attribute__((aligned(16))) char lds_big[];
+    const unsigned int nbig = a.big_count[round];
+    if (nbig == 0) return;
+    if (a.ctr[kCtrStatus] != 0ull) return;
+    Cursor fcur, ocur;
+    long long *ws = a.wave_state + 4 * ((size_t)a.big_ws + blockIdx.x);
+    fcur.cur = ws[0]; fcur.end = ws[1];
+    ocur.cur = ws[2]; ocur.end = ws[3];
+    for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const int col = a.big_list[i];
+        if (!eliminate_column(a, col, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr)) break;
+    }
     if (lane_id() == 0) {
         ws[0] = fcur.cur; ws[1] = fcur.end;
         ws[2] = ocur.cur; ws[3] = ocur.end;
@@ -399,12 +428,20 @@ void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, 
         const int *cur = (r & 1) ? a.list1 : a.list0;
         int *next = (r & 1) ? a.list0 : a.list1;
         hipLaunchKernelGGL(k_factor_round, dim3(num_blocks), dim3(kWave), 0, s, a, r, cur, next);
+        hipLaunchKernelGGL(k_factor_round_big, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, a, r,
+                           next);
     }
+}
+
+void factor_kernel_setup()
+{
+    (void)hipFuncSetAttribute((const void *)k_factor_round_big, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)kBigCap * kBytesPerEntry));
 }
 
 int factor_lds_capacity() { return kLdsCap; }
 int factor_scratch_bytes_per_entry() { return kBytesPerEntry; }
-long long factor_pool_slack(int num_blocks) { return (long long)num_blocks * (kFillChunk + kOutChunk); }
+long long factor_pool_slack(int num_blocks) { return (long long)(num_blocks + kBigGrid) * (kFillChunk + kOutChunk); }
 
 int factor_occupancy_blocks(int device)
 {

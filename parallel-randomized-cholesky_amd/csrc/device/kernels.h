@@ -37,6 +37,12 @@ enum : unsigned long long {
 // kFillLists)), so a column's gather walks them in parallel.
 constexpr int kFillLists = 16;
 
+// Columns with more than the round kernel's LDS capacity (128 entries) are
+// deferred to a second kernel of the same round: kBigGrid one-wave workgroups
+// with kBigCap entries of LDS each.
+constexpr int kBigGrid = 256;
+constexpr int kBigCap = 2048;
+
 struct FactorArgs {
     int n;
     const int *lo_ptr;      // n+1, strict-lower part of the Laplacian (CSC)
@@ -55,7 +61,10 @@ struct FactorArgs {
     long long *out_start;
     int *out_cnt;
     double *diag;
-    char *scratch;          // per-workgroup overflow buffers
+    int *big_list;          // columns deferred in the current round
+    unsigned int *big_count;   // [round] = deferred columns
+    int big_ws;             // wave_state index of the first large-LDS workgroup
+    char *scratch;          // overflow buffers of the kBigGrid deferred-column workgroups
     int scratch_cap;        // entries per workgroup (power of two)
     uint64_t seed;
 };
@@ -67,6 +76,7 @@ int factor_lds_capacity();
 int factor_scratch_bytes_per_entry();
 long long factor_pool_slack(int num_blocks);
 int factor_occupancy_blocks(int device);
+void factor_kernel_setup();   // large dynamic LDS of the deferred-column kernel
 
 // ------------------------------------------------------------ assembly
 // Exclusive scan of n int32 counts into out[0..n] (out[n] = total, so `out`

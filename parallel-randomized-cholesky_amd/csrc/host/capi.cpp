@@ -86,6 +86,7 @@ struct rc_handle_s {
 
     // factor inputs / working set
     DevVec lo_ptr, lo_row, lo_w, dep_init;
+    DevVec big_list, big_count;
     DevVec dep, head, nfill, level, list0, list1, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
         out_cnt, diag64, scratch;
     long long pool_cap = 0, out_cap = 0;
@@ -244,6 +245,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
         rate_khz = 100000;
     h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
     h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
+    factor_kernel_setup();
     h->sweep_blocks = sweep_occupancy_blocks(o.device);
     h->sweep_blocks_max = h->sweep_blocks;
     {
@@ -721,7 +723,9 @@ static int alloc_factor_work(rc_handle_t h)
     RC_HIP(h->list0.ensure(nf * sizeof(int)));
     RC_HIP(h->list1.ensure(nf * sizeof(int)));
     RC_HIP(h->round_count.ensure((nf + 2) * sizeof(unsigned int)));
-    RC_HIP(h->wave_state.ensure((size_t)h->num_blocks * 4 * sizeof(long long)));
+    RC_HIP(h->big_count.ensure((nf + 2) * sizeof(unsigned int)));
+    RC_HIP(h->big_list.ensure(nf * sizeof(int)));
+    RC_HIP(h->wave_state.ensure((size_t)(h->num_blocks + kBigGrid) * 4 * sizeof(long long)));
     RC_HIP(h->out_start.ensure(nf * sizeof(long long)));
     RC_HIP(h->out_cnt.ensure(nf * sizeof(int)));
     RC_HIP(h->diag64.ensure(nf * sizeof(double)));
@@ -729,7 +733,8 @@ static int alloc_factor_work(rc_handle_t h)
     RC_HIP(h->pool.ensure((size_t)h->pool_cap * sizeof(FillRec)));
     RC_HIP(h->out_row.ensure((size_t)h->out_cap * sizeof(int)));
     RC_HIP(h->out_val.ensure((size_t)h->out_cap * sizeof(double)));
-    RC_HIP(h->scratch.ensure((size_t)h->num_blocks * (size_t)h->scratch_cap * (size_t)factor_scratch_bytes_per_entry()));
+    // only the deferred-column kernel (kBigGrid workgroups) uses the scratch
+    RC_HIP(h->scratch.ensure((size_t)kBigGrid * (size_t)h->scratch_cap * (size_t)factor_scratch_bytes_per_entry()));
     return RC_OK;
 }
 
@@ -757,6 +762,9 @@ static FactorArgs factor_args(rc_handle_t h)
     a.out_start = h->out_start.as<long long>();
     a.out_cnt = h->out_cnt.as<int>();
     a.diag = h->diag64.as<double>();
+    a.big_list = h->big_list.as<int>();
+    a.big_count = h->big_count.as<unsigned int>();
+    a.big_ws = h->num_blocks;
     a.scratch = h->scratch.as<char>();
     a.scratch_cap = h->scratch_cap;
     a.seed = h->opt.seed;
@@ -769,7 +777,7 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
     if (h->pool_cap == 0) {
         h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low)) + slack;
         h->out_cap = h->pool_cap + h->nnz_low + slack;
-        h->scratch_cap = 2048;
+        h->scratch_cap = 8192;
     }
     *regrowths = 0;
     hipStream_t s = h->stream;
@@ -783,7 +791,8 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         RC_HIP(hipEventRecord(e0, s));
         RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), s));
         RC_HIP(hipMemsetAsync(h->round_count.p, 0, ((size_t)h->nf + 2) * sizeof(unsigned int), s));
-        RC_HIP(hipMemsetAsync(h->wave_state.p, 0, (size_t)h->num_blocks * 4 * sizeof(long long), s));
+        RC_HIP(hipMemsetAsync(h->big_count.p, 0, ((size_t)h->nf + 2) * sizeof(unsigned int), s));
+        RC_HIP(hipMemsetAsync(h->wave_state.p, 0, (size_t)(h->num_blocks + kBigGrid) * 4 * sizeof(long long), s));
         launch_factor_setup(a, h->dep_init.as<int>(), s);
         // rounds in growing batches; a round whose list is empty returns at once
         int round = 0, batch = 8;
@@ -1361,7 +1370,7 @@ int rc_load_factor(rc_handle_t h, const char *prefix)
     }
     if (h->pool_cap == 0) {
         h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low));
-        h->scratch_cap = 2048;
+        h->scratch_cap = 8192;
     }
     h->out_cap = std::max<long long>(h->out_cap, tot + 16);
     rc = alloc_factor_work(h);

@@ -35,15 +35,18 @@ def _case(wl, name):
         return wl.grid_sddm((7, 7, 7)), "sddm"
     if name == "two_comp":
         return read_mtx_py(golden_path("two_components.mtx")), "laplacian"
-    if name == "star_300":
-        d = 300
+    if name.startswith("star_"):
+        # centre eliminated first: one column with d entries.  300 uses the
+        # large-LDS kernel, 3000 its global scratch, 9000 a scratch regrowth
+        d = int(name.split("_")[1])
         u = np.zeros(d, np.int64); v = np.arange(1, d + 1)
         w = np.linspace(0.5, 3.0, d)
         return wl.laplacian_triplets(d + 1, u, v, w), "laplacian_asis"
     raise KeyError(name)
 
 
-CASES = ["grid2d_16_log", "grid3d_8", "grid3d_20_log", "rgg_3000", "sddm_7", "two_comp", "star_300"]
+CASES = ["grid2d_16_log", "grid3d_8", "grid3d_20_log", "rgg_3000", "sddm_7", "two_comp", "star_300", "star_3000",
+         "star_9000"]
 
 
 def _solver(rc, wl, name, prec=64, seed=11, **kw):
