@@ -29,6 +29,8 @@ def _case(wl, name):
         return wl.grid_laplacian((8, 8, 8)), "laplacian"
     if name == "grid3d_20_log":
         return wl.grid_laplacian((20, 20, 20), "loguniform", seed=7), "laplacian"
+    if name == "grid3d_64_log":
+        return wl.grid_laplacian((64, 64, 64), "loguniform", seed=7), "laplacian"
     if name == "rgg_3000":
         return wl.random_geometric_graph(3000, seed=3), "laplacian"
     if name == "sddm_7":
@@ -149,11 +151,12 @@ def test_schedule_independence(rc, wl):
 
 def test_pool_regrowth_gives_same_factor(rc, wl):
     say("[gpu] pool regrowth")
-    s1, _, _ = _solver(rc, wl, "grid3d_20_log", 64)
+    s1, _, _ = _solver(rc, wl, "grid3d_64_log", 64)
     s1.factor()
-    # one workgroup keeps the per-wave chunk slack of the pools small, so a
-    # tiny pool factor must overflow and regrow
-    s2, _, _ = _solver(rc, wl, "grid3d_20_log", 64, pool_factor=0.05, num_blocks=1)
+    # one workgroup keeps the per-wave chunk slack of the pools small (the
+    # deferred-column grid adds about a million entries), so a tiny pool
+    # factor must overflow on 2.2M fills and regrow
+    s2, _, _ = _solver(rc, wl, "grid3d_64_log", 64, pool_factor=0.05, num_blocks=1)
     st2 = s2.factor()
     assert st2["pool_regrowths"] >= 1
     for a, b in zip(s1.get_factor(), s2.get_factor()):
