@@ -302,6 +302,29 @@ int64_t orc_permute(int n, const int64_t *colptr, const int *rowind, const doubl
 /* ------------------------------------------------------------------------ */
 typedef struct { int row; int next; double w; } fill_t;
 
+/* Forward cumulative sums of e[0..d).w in the canonical blocked order shared
+ * with the HIP kernel: blocks of 64 entries; inside a block the doubling scan
+ * (offsets 1, 2, ..., 32, every step from the previous step's values: x_i +=
+ * x_{i-off}); then the previous block's last sum is added.  The reference sums
+ * sequentially on the CPU (driver_local.cpp:573-593) and with a CUB scan on
+ * the GPU (driver.cu:931); any fixed order restates it, this one lets a wave
+ * compute it in six shuffle steps and still match the oracle bit for bit. */
+static void canonical_scan(const entry_t *e, int64_t d, double *C)
+{
+    double x[64], y[64];
+    double carry = 0.0;
+    for (int64_t b = 0; b < d; b += 64) {
+        const int nb = (int)(d - b < 64 ? d - b : 64);
+        for (int i = 0; i < 64; ++i) x[i] = i < nb ? e[b + i].w : 0.0;
+        for (int off = 1; off < 64; off <<= 1) {
+            for (int i = 0; i < 64; ++i) y[i] = i >= off ? x[i] + x[i - off] : x[i];
+            memcpy(x, y, sizeof x);
+        }
+        for (int i = 0; i < nb; ++i) C[b + i] = b == 0 ? x[i] : carry + x[i];
+        carry = C[b + nb - 1];
+    }
+}
+
 int orc_factor(int n, const int64_t *colptr, const int *rowind, const double *val,
                uint64_t seed, int mutation, int64_t cap,
                int64_t *lt_ptr, int *lt_ind, double *lt_val, double *diag, int *level,
@@ -362,8 +385,7 @@ int orc_factor(int n, const int64_t *colptr, const int *rowind, const double *va
         if (d > s.max_degree) s.max_degree = d;
         /* 3. sort by (value, row); forward cumulative sums (:573-593) */
         qsort(e, (size_t)d, sizeof(entry_t), cmp_value_row);
-        C[0] = e[0].w;
-        for (int64_t i = 1; i < d; ++i) C[i] = C[i - 1] + e[i].w;
+        canonical_scan(e, d, C);
         double S = C[d - 1];
         /* 4. one sampled fill per entry except the last (:598-631,
          *    gpu auxilliary.hpp:543-592) */

@@ -244,12 +244,21 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, int lds_
     }
     __syncthreads();
     wave_sort<false>(b.srow, b.sw, b.smult, d);
-    if (lane == 0) {
-        double c = b.sw[0];
-        b.C[0] = c;
-        for (int i = 1; i < d; ++i) {
-            c = c + b.sw[i];
-            b.C[i] = c;
+    // canonical blocked scan (shared with the oracle): per 64-entry block the
+    // doubling scan x_i += x_{i-off}, off = 1..32, then the previous block's
+    // last sum is added
+    {
+        double carry = 0.0;
+        for (int base_i = 0; base_i < d; base_i += kWave) {
+            const int i = base_i + lane;
+            double x = i < d ? b.sw[i] : 0.0;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const double o = __shfl_up(x, off, kWave);
+                if (lane >= off) x = x + o;
+            }
+            const double c = base_i == 0 ? x : carry + x;
+            if (i < d) b.C[i] = c;
+            carry = __shfl(c, kWave - 1, kWave);
         }
     }
     __syncthreads();
