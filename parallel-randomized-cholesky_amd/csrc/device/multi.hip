@@ -108,29 +108,59 @@ __global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p_old,
 }
 
 // ---------------------------------------------------------------- sweeps
-// Persistent: a block claims a chunk of the level order and walks its rows,
-// rows_per_block at a time (K lanes per row).
-template <typename T, bool kFwd>
-__global__ void __launch_bounds__(kMB) k_msweep(MultiArgs a, const double *p_cur, int update_rx)
+// Persistent: a block claims a chunk (<= 64 rows of one level) and solves its
+// rows R at a time: L = K / V lanes per row, each lane V right-hand sides (V =
+// 4 in FP32, 2 in FP64), so a gather of row j's values is one 16-byte access
+// per lane; R = min(64, 1024 / L), so FP32 chunks take a single pass.  Entry
+// indices and values are loaded eight at a time, then the eight gathers.
+template <typename T, int V>
+__device__ __forceinline__ void vload(const T *p, T *v)
+{
+    if constexpr (V == 4 && sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else if constexpr (V >= 2 && sizeof(T) == 8) {
+#pragma unroll
+        for (int h = 0; h < V; h += 2) {
+            const double2 q = *reinterpret_cast<const double2 *>(p + h);
+            v[h] = q.x; v[h + 1] = q.y;
+        }
+    } else if constexpr (V == 2) {
+        const float2 q = *reinterpret_cast<const float2 *>(p);
+        v[0] = q.x; v[1] = q.y;
+    } else {
+        v[0] = *p;
+    }
+}
+
+template <typename T, bool kFwd, int V>
+__global__ void __launch_bounds__(1024) k_msweep(MultiArgs a, const double *p_cur, int update_rx)
 {
     __shared__ int s_chunk, s_stop;
     MultiScalars *sc = a.sc;
     if (sc->all_done) return;
     const int K = a.K;
-    const int q = threadIdx.x % K;
-    const int sub = threadIdx.x / K;
-    const int rows_per_block = kMB / K;
-    const bool live = q < a.k;
-    const double alpha = (kFwd && update_rx && live && !sc->done[q]) ? sc->alpha[q] : 0.0;
+    const int L = K / V;
+    const int t = threadIdx.x;
+    const int i0 = t / L, g = t - i0 * L;
+    const int R = blockDim.x / L;
+    const int q0 = g * V;
+    double alpha[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int q = q0 + v;
+        alpha[v] = (kFwd && update_rx && q < a.k && !sc->done[q]) ? sc->alpha[q] : 0.0;
+    }
     T *y = (T *)a.y;
     T *z = (T *)a.z;
+    T *vec = kFwd ? y : z;
     const T *val = (const T *)(kFwd ? a.l_val : a.lt_val);
     const int *ptr = kFwd ? a.l_ptr : a.lt_ptr;
     const int *ind = kFwd ? a.l_ind : a.lt_ind;
     const T *diag = (const T *)a.diag;
     unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
     while (true) {
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             s_chunk = (int)atomicAdd(ctr, 1u);
             s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
         }
@@ -141,54 +171,65 @@ __global__ void __launch_bounds__(kMB) k_msweep(MultiArgs a, const double *p_cur
         if (c >= a.nchunks || stop) break;
         if (!kFwd) c = a.nchunks - 1 - c;
         const int cnt = a.chunk_cnt[c];
-        const int p0 = a.chunk_start[c];
-        for (int i = sub; i < cnt; i += rows_per_block) {
-            const int k = p0 + i;
-            const size_t o = (size_t)k * K + q;
-            T acc;
-            if (kFwd) {
-                double rk = a.r[o];
-                if (update_rx && alpha != 0.0) {
-                    rk = fma_t(-alpha, a.ap[o], rk);
-                    a.r[o] = rk;
-                    a.x[o] = fma_t(alpha, p_cur[o], a.x[o]);
+        for (int i = i0; i < cnt; i += R) {
+        const int k = a.chunk_start[c] + i;
+        const size_t o = (size_t)k * K + q0;
+        T acc[V];
+        if (kFwd) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                double rk = a.r[o + v];
+                if (alpha[v] != 0.0) {
+                    rk = fma_t(-alpha[v], a.ap[o + v], rk);
+                    a.r[o + v] = rk;
+                    a.x[o + v] = fma_t(alpha[v], p_cur[o + v], a.x[o + v]);
                 }
-                acc = (T)rk;
-            } else {
-                T yk = y[o];
-                y[o] = sent_value(T());
-                T dk = diag[k];
-                if (dk != (T)0) yk = yk / dk;
-                acc = yk;
+                acc[v] = (T)rk;
             }
-            T *vec = kFwd ? y : z;
-            const int e1 = ptr[k + 1];
-            int e = ptr[k];
-            for (; e + 4 <= e1; e += 4) {
-                int j[4];
-                T l[4], v[4];
+        } else {
+            T yv[V];
+            vload<T, V>(y + o, yv);
+            const T dk = diag[k];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) { j[u] = ind[e + u]; l[u] = val[e + u]; }
+            for (int v = 0; v < V; ++v) {
+                y[o + v] = sent_value(T());
+                acc[v] = dk != (T)0 ? yv[v] / dk : yv[v];
+            }
+        }
+        const int e1 = ptr[k + 1];
+        int e = ptr[k];
+        constexpr int B = 8;
+        for (; e < e1; e += B) {
+            int j[B];
+            T l[B];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = vec[(size_t)j[u] * K + q];
+            for (int u = 0; u < B; ++u) {
+                const bool in = e + u < e1;
+                j[u] = in ? ind[e + u] : a.n;            // dummy slot n holds zeros
+                l[u] = in ? val[e + u] : (T)0;
+            }
+            T w[B][V];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (is_sent(v[u])) v[u] = mwait_slow(vec + (size_t)j[u] * K + q, a.spin_ticks, sc);
-                    acc = fma_t(-l[u], v[u], acc);
+            for (int u = 0; u < B; ++u) vload<T, V>(vec + (size_t)j[u] * K + q0, w[u]);
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    T x = w[u][v];
+                    if (is_sent(x)) x = mwait_slow(vec + (size_t)j[u] * K + q0 + v, a.spin_ticks, sc);
+                    acc[v] = fma_t(-l[u], x, acc[v]);
                 }
             }
-            for (; e < e1; ++e) {
-                const int j = ind[e];
-                T v = vec[(size_t)j * K + q];
-                if (is_sent(v)) v = mwait_slow(vec + (size_t)j * K + q, a.spin_ticks, sc);
-                acc = fma_t(-val[e], v, acc);
-            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
             if (kFwd) {
-                st_relaxed_val(y + o, unsent(acc));
-                z[o] = sent_value(T());
+                st_relaxed_val(y + o + v, unsent(acc[v]));
+                z[o + v] = sent_value(T());
             } else {
-                st_relaxed_val(z + o, unsent(acc));
+                st_relaxed_val(z + o + v, unsent(acc[v]));
             }
+        }
         }
     }
 }
@@ -362,6 +403,25 @@ int msweep_blocks(const MultiArgs &a)
     return g > 0 ? g : 1;
 }
 
+// One sweep: 64 rows x (K / V) lanes per block, V = 4 right-hand sides per lane
+// (fewer when K < 4).
+template <typename T, bool kFwd>
+void launch_msweep(const MultiArgs &a, const double *p_cur, int update_rx, hipStream_t s)
+{
+    const int vmax = sizeof(T) == 4 ? 4 : 2;
+    const int V = a.K >= vmax ? vmax : a.K;
+    const int L = a.K / V;
+    const dim3 block((L * 64 <= 1024 ? 64 : 1024 / L) * L);
+    if constexpr (sizeof(T) == 4) {
+        if (V == 4) {
+            hipLaunchKernelGGL((k_msweep<T, kFwd, 4>), dim3(msweep_blocks(a)), block, 0, s, a, p_cur, update_rx);
+            return;
+        }
+    }
+    if (V == 2) hipLaunchKernelGGL((k_msweep<T, kFwd, 2>), dim3(msweep_blocks(a)), block, 0, s, a, p_cur, update_rx);
+    else hipLaunchKernelGGL((k_msweep<T, kFwd, 1>), dim3(msweep_blocks(a)), block, 0, s, a, p_cur, update_rx);
+}
+
 }  // namespace
 
 template <typename T>
@@ -369,8 +429,8 @@ void launch_multi_init(const MultiArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mprepare<T>, dim3(2048), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_mresidual, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 1);
-    hipLaunchKernelGGL((k_msweep<T, true>), dim3(msweep_blocks(a)), dim3(kMB), 0, s, a, (const double *)a.p0, 0);
-    hipLaunchKernelGGL((k_msweep<T, false>), dim3(msweep_blocks(a)), dim3(kMB), 0, s, a, (const double *)nullptr, 0);
+    launch_msweep<T, true>(a, (const double *)a.p0, 0, s);
+    launch_msweep<T, false>(a, (const double *)nullptr, 0, s);
     hipLaunchKernelGGL(k_mdot<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 1);
 }
 
@@ -380,8 +440,8 @@ void launch_multi_iteration(const MultiArgs &a, int parity, hipStream_t s)
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
     hipLaunchKernelGGL(k_mspmv<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, (const double *)p_old, p_new);
-    hipLaunchKernelGGL((k_msweep<T, true>), dim3(msweep_blocks(a)), dim3(kMB), 0, s, a, (const double *)p_new, 1);
-    hipLaunchKernelGGL((k_msweep<T, false>), dim3(msweep_blocks(a)), dim3(kMB), 0, s, a, (const double *)nullptr, 0);
+    launch_msweep<T, true>(a, (const double *)p_new, 1, s);
+    launch_msweep<T, false>(a, (const double *)nullptr, 0, s);
     hipLaunchKernelGGL(k_mdot<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 0);
 }
 
