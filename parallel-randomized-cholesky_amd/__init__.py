@@ -40,6 +40,7 @@ EXPORTS = (
     "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
     "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
     "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats", "rc_autotune",
+    "rc_set_rhs_multi", "rc_solve_multi_resident", "rc_get_solution_multi",
 )
 
 
@@ -145,6 +146,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_solve_batch.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.rc_get_batch_stats.argtypes = [vp, vp]
     lib.rc_autotune.argtypes = [vp, ctypes.c_int, P(TuneStats)]
+    lib.rc_set_rhs_multi.argtypes = [vp, ctypes.c_int, vp]
+    lib.rc_solve_multi_resident.argtypes = [vp, ctypes.c_double, vp]
+    lib.rc_get_solution_multi.argtypes = [vp, vp]
     _lib = lib
     return lib
 
@@ -347,6 +351,26 @@ class Solver:
         if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
             self._check(rc)
         return Xc.T.copy(), [st[q].as_dict() for q in range(k)]
+
+    def set_rhs_multi(self, B):
+        """Upload k right-hand sides (columns of B, shape (n, k)) once."""
+        B = np.asarray(B, dtype=np.float64)
+        self._Bc = np.ascontiguousarray(B.T)          # column-major for the C ABI
+        self._k = B.shape[1]
+        self._check(self.lib.rc_set_rhs_multi(self.h, self._k, _ptr(self._Bc)))
+
+    def solve_multi_resident(self, tol=1e-6, allow_breakdown=False):
+        st = (SolveStats * self._k)()
+        rc = self.lib.rc_solve_multi_resident(self.h, float(tol), st)
+        if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
+            self._check(rc)
+        return [st[q].as_dict() for q in range(self._k)]
+
+    def get_solution_multi(self):
+        n, _ = self.size()
+        Xc = np.empty((self._k, n), dtype=np.float64)
+        self._check(self.lib.rc_get_solution_multi(self.h, _ptr(Xc)))
+        return Xc.T.copy()
 
     def set_rhs(self, b):
         self._b = np.ascontiguousarray(b, dtype=np.float64)

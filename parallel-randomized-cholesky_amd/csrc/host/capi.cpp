@@ -141,7 +141,9 @@ struct rc_handle_s {
     // multi-RHS: CSRs in position numbering (built lazily per factor) + vectors
     bool multi_ready = false;
     DevVec m_aptr, m_aind, m_aval, m_lptr, m_lind, m_lval, m_ltptr, m_ltind, m_ltval;
-    DevVec m_x, m_r, m_ap, m_p0, m_p1, m_y, m_z, m_b, m_B, m_sc, m_part;
+    DevVec m_x, m_r, m_ap, m_p0, m_p1, m_y, m_z, m_b, m_B, m_X, m_sc, m_part;
+    int m_k = 0;                    // right-hand sides held in m_B (user order, column-major)
+    bool m_solved = false;
 
     ~rc_handle_s()
     {
@@ -1847,17 +1849,8 @@ static int build_multi(rc_handle_t h)
 
 extern "C" {
 
-int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
+static int multi_args(rc_handle_t h, int k, MultiArgs &a)
 {
-    RC_CHECK_HANDLE();
-    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
-    if (k <= 0 || k > kMaxRhs || !B || !X || !(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
-    if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_solve_multi");
-    if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
-    RC_HIP(hipSetDevice(h->opt.device));
-    int rc = h->prec == 32 ? build_multi<float>(h) : build_multi<double>(h);
-    if (rc != RC_OK) return rc;
-    hipStream_t s = h->stream;
     int K = 1;
     while (K < k) K <<= 1;
     const size_t n = (size_t)h->ns;
@@ -1871,11 +1864,9 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
     RC_HIP(h->m_b.ensure(tot * 8));
     RC_HIP(h->m_y.ensure(tot * sz));
     RC_HIP(h->m_z.ensure(tot * sz));
-    RC_HIP(h->m_B.ensure(n * (size_t)k * 8));
     RC_HIP(h->m_sc.ensure(sizeof(MultiScalars)));
     RC_HIP(h->m_part.ensure((size_t)h->reduce_blocks * 2 * K * sizeof(double)));
     RC_HIP(h->ctrs.ensure(4 * sizeof(unsigned int)));
-    MultiArgs a;
     a.n = (int)n;
     a.K = K;
     a.k = k;
@@ -1907,19 +1898,53 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
     a.reduce_blocks = h->reduce_blocks;
     a.sweep_blocks = h->sweep_blocks;
     a.spin_ticks = h->spin_ticks;
+    return RC_OK;
+}
 
+int rc_set_rhs_multi(rc_handle_t h, int k, const double *B)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (k <= 0 || k > kMaxRhs || !B) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
+    RC_HIP(hipSetDevice(h->opt.device));
+    const size_t n = (size_t)h->n_user;
+    RC_HIP(h->m_B.ensure(n * (size_t)k * 8));
+    RC_HIP(hipMemcpyAsync(h->m_B.p, B, n * (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    h->m_k = k;
+    return RC_OK;
+}
+
+int rc_solve_multi_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (!(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad tolerance");
+    if (!h->factored || h->m_k <= 0) return fail(h, RC_ERR_STATE, "rc_factor and rc_set_rhs_multi must precede");
+    if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
+    if ((size_t)h->ns != (size_t)h->n_user) return fail(h, RC_ERR_STATE, "right-hand sides do not match the system");
+    RC_HIP(hipSetDevice(h->opt.device));
+    hipStream_t s = h->stream;
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    RC_HIP(hipEventRecord(e0, s));
+    int rc = h->prec == 32 ? build_multi<float>(h) : build_multi<double>(h);
+    if (rc != RC_OK) return rc;
+    const int k = h->m_k;
+    MultiArgs a;
+    rc = multi_args(h, k, a);
+    if (rc != RC_OK) return rc;
+    const size_t n = (size_t)h->ns;
     MultiScalars sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.tol = tol;
     sc.maxit = h->opt.maxit;
     sc.stop_rule = h->opt.stop_rule;
-    RC_HIP(hipMemcpyAsync(h->m_B.p, B, n * (size_t)k * 8, hipMemcpyHostToDevice, s));
     RC_HIP(hipMemcpyAsync(h->m_sc.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
-    launch_multi_gather(h->m_B.as<double>(), h->idx_user.as<int>(), (int)n, k, K, h->m_b.as<double>(), s);
-    hipEvent_t e0, e1;
-    RC_HIP(hipEventCreate(&e0));
-    RC_HIP(hipEventCreate(&e1));
-    RC_HIP(hipEventRecord(e0, s));
+    launch_multi_gather(h->m_B.as<double>(), h->idx_user.as<int>(), (int)n, k, a.K, h->m_b.as<double>(), s);
     if (h->prec == 32) launch_multi_init<float>(a, s); else launch_multi_init<double>(a, s);
     int *done_h = nullptr;
     RC_HIP(hipHostMalloc((void **)&done_h, sizeof(int) * 2, hipHostMallocDefault));
@@ -1938,16 +1963,15 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
         if (done_h[0]) break;
     }
     (void)hipHostFree(done_h);
-    RC_HIP(hipEventRecord(e1, s));
     launch_multi_residual(a, s);
+    RC_HIP(hipEventRecord(e1, s));
     RC_HIP(hipMemcpyAsync(&sc, h->m_sc.p, sizeof(sc), hipMemcpyDeviceToHost, s));
-    launch_multi_scatter(a.x, h->idx_user.as<int>(), (int)n, k, K, h->m_B.as<double>(), s);
-    RC_HIP(hipMemcpyAsync(X, h->m_B.p, n * (size_t)k * 8, hipMemcpyDeviceToHost, s));
     RC_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    h->m_solved = true;
     int any_break = 0;
     for (int q = 0; q < k; ++q) {
         any_break |= sc.breakdown[q];
@@ -1965,6 +1989,36 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
     if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
     if (any_break) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown in at least one right-hand side");
     return RC_OK;
+}
+
+int rc_get_solution_multi(rc_handle_t h, double *X)
+{
+    RC_CHECK_HANDLE();
+    if (!X) return fail(h, RC_ERR_ARG, "null X");
+    if (!h->m_solved || h->m_k <= 0) return fail(h, RC_ERR_STATE, "no multi-RHS solve");
+    RC_HIP(hipSetDevice(h->opt.device));
+    const int k = h->m_k;
+    int K = 1;
+    while (K < k) K <<= 1;
+    const size_t n = (size_t)h->ns;
+    RC_HIP(h->m_X.ensure(n * (size_t)k * 8));
+    launch_multi_scatter(h->m_x.as<double>(), h->idx_user.as<int>(), (int)n, k, K, h->m_X.as<double>(), h->stream);
+    RC_HIP(hipMemcpyAsync(X, h->m_X.p, n * (size_t)k * 8, hipMemcpyDeviceToHost, h->stream));
+    RC_HIP(hipStreamSynchronize(h->stream));
+    return RC_OK;
+}
+
+int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (!B || !X) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
+    if (!h->factored) return fail(h, RC_ERR_STATE, "rc_factor must precede rc_solve_multi");
+    int rc = rc_set_rhs_multi(h, k, B);
+    if (rc != RC_OK) return rc;
+    rc = rc_solve_multi_resident(h, tol, st);
+    if (rc != RC_OK && rc != RC_ERR_BREAKDOWN) return rc;
+    const int rc2 = rc_get_solution_multi(h, X);
+    return rc2 != RC_OK ? rc2 : rc;
 }
 
 // ------------------------------------------------------------------ timing

@@ -340,3 +340,20 @@ def test_batch_matches_oracle_per_system(rc, orc, wl, prec, many):
     tot = s.solve_resident(1e-6)
     assert tot["iterations"] == max(its)
     assert s.batch_stats()[1]["iterations"] == sts[1]["iterations"]
+
+
+def test_multi_rhs_resident_matches_one_shot(rc, wl):
+    say("[gpu] multi-rhs resident")
+    s, data, kind = _solver(rc, wl, "grid3d_20_log", 32)
+    s.factor()
+    B = np.stack([s.make_rhs(rc.RHS_NORMAL_PROJECTED, 7 + q) for q in range(3)], axis=1)
+    X1, st1 = s.solve_multi(B, tol=1e-6)
+    s.set_rhs_multi(B)
+    st2 = s.solve_multi_resident(1e-6)
+    X2 = s.get_solution_multi()
+    assert [d["iterations"] for d in st1] == [d["iterations"] for d in st2]
+    np.testing.assert_array_equal(X1, X2)
+    s.factor()                                      # a new factor reuses the resident right-hand sides
+    st3 = s.solve_multi_resident(1e-6)
+    np.testing.assert_array_equal(s.get_solution_multi(), X2)
+    assert [d["iterations"] for d in st3] == [d["iterations"] for d in st2]
