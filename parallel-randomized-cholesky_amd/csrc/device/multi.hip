@@ -44,9 +44,25 @@ __device__ __noinline__ T mwait_slow(T *p, long long spin_ticks, MultiScalars *s
 }
 
 // ---------------------------------------------------------------- SpMM
-// p_new = z + beta_q p_old ; Ap = A p_new ; partial p'Ap per q.
+// p_new = z + beta_q p_old (streaming); then Ap = A p_new and per-q partials
+// of p'Ap.
 template <typename T>
-__global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p_old, double *p_new)
+__global__ void __launch_bounds__(kMB) k_mpupdate(MultiArgs a, const double *p_old, double *p_new)
+{
+    MultiScalars *sc = a.sc;
+    if (sc->all_done) return;
+    const int K = a.K;
+    const T *z = (const T *)a.z;
+    const size_t total = (size_t)a.n * K;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int q = (int)(i & (size_t)(K - 1));
+        const double beta = q < a.k ? sc->beta[q] : 0.0;
+        p_new[i] = fma_t(beta, p_old[i], (double)z[i]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p)
 {
     __shared__ double red[kMB];
     __shared__ int flag;
@@ -55,22 +71,26 @@ __global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p_old,
     if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
     const int K = a.K;
     const int q = threadIdx.x % K;
-    const double beta = q < a.k ? sc->beta[q] : 0.0;
-    const T *z = (const T *)a.z;
     const int rows_per_block = kMB / K;
     double acc_dot = 0.0;
     for (int row = blockIdx.x * rows_per_block + threadIdx.x / K; row < a.n; row += gridDim.x * rows_per_block) {
         double acc = 0.0;
-        for (int e = a.a_ptr[row]; e < a.a_ptr[row + 1]; ++e) {
-            const int j = a.a_ind[e];
-            const double pj = beta * p_old[(size_t)j * K + q] + (double)z[(size_t)j * K + q];
-            acc = fma_t(a.a_val[e], pj, acc);
+        const int e1 = a.a_ptr[row + 1];
+        int e = a.a_ptr[row];
+        for (; e + 4 <= e1; e += 4) {
+            int j[4];
+            double v[4], pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { j[u] = a.a_ind[e + u]; v[u] = a.a_val[e + u]; }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = p[(size_t)j[u] * K + q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
         }
+        for (; e < e1; ++e) acc = fma_t(a.a_val[e], p[(size_t)a.a_ind[e] * K + q], acc);
         const size_t o = (size_t)row * K + q;
-        const double pi = beta * p_old[o] + (double)z[o];
-        p_new[o] = pi;
         a.ap[o] = acc;
-        acc_dot += pi * acc;
+        acc_dot += p[o] * acc;
     }
     // per-q block partial: threads t, t+K, t+2K, ... share q
     red[threadIdx.x] = acc_dot;
@@ -439,7 +459,8 @@ void launch_multi_iteration(const MultiArgs &a, int parity, hipStream_t s)
 {
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
-    hipLaunchKernelGGL(k_mspmv<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, (const double *)p_old, p_new);
+    hipLaunchKernelGGL(k_mpupdate<T>, dim3(4096), dim3(kMB), 0, s, a, (const double *)p_old, p_new);
+    hipLaunchKernelGGL(k_mspmv<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, (const double *)p_new);
     launch_msweep<T, true>(a, (const double *)p_new, 1, s);
     launch_msweep<T, false>(a, (const double *)nullptr, 0, s);
     hipLaunchKernelGGL(k_mdot<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 0);
