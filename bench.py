@@ -78,6 +78,18 @@ def make_batch(wl, cfg, count=None):
     return [wl.random_geometric_graph(m, seed=1000 + i) for i, m in enumerate(sizes)]
 
 
+def traffic_of(workload, group):
+    """Bytes per launch leaving the L2 for the kernel group, from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload
+    (profiles/traffic_<workload>.json, scaled by the copy-kernel calibration);
+    None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    g = json.load(open(path)).get("groups", {}).get(group)
+    return g["traffic_bytes"] if g else None
+
+
 def make_system(wl, cfg):
     if cfg["kind"] == "sddm":
         return wl.grid_sddm(cfg["dims"], 1e-2)
@@ -275,7 +287,8 @@ def main():
     per_solve = {k: kern[k]["ms"] * (np.mean(iters) + 1) for k in ("spmv", "trsv_fwd", "trsv_bwd")}
     per_solve["factor"] = fst["factor_ms"]
     dom = max(per_solve, key=per_solve.get)
-    kname = {"spmv": "k_spmv_plain", "trsv_fwd": "k_trsv_fwd", "trsv_bwd": "k_trsv_bwd", "factor": "k_factor"}[dom]
+    kname = {"spmv": "k_spmv_plain", "trsv_fwd": "forward sweep (k_sweep_bulk + k_tail_prep + k_tail_solve)",
+             "trsv_bwd": "backward sweep (k_tail_solve + k_sweep_bulk)", "factor": "k_factor_round (all rounds)"}[dom]
     achieved = kern[dom]["GBps"]
 
     x = s.get_solution()
@@ -297,7 +310,7 @@ def main():
                    "systems": s.batch()[0], "tol": args.tol,
                    "stop_rule": "reference sqrt(r'z/r0'z0)"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_of(cfg["workload"], dom)},
         "detail": {
             "wall_s_timed_loop": wall,
             "factor_ms": float(np.mean(fms)), "assemble_ms": float(np.mean(ams)), "pcg_ms": float(np.mean(pms)),

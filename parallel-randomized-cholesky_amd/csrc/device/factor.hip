@@ -128,8 +128,8 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
 
 // Eliminates column k; newly ready neighbours are appended to `next`.
 // Returns false if the pools or the scratch are exhausted (status set).
-__device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, int lds_cap, Cursor &fcur, Cursor &ocur,
-                                 int *next, unsigned int *next_count, unsigned int *defer_count)
+__device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
+                                 Cursor &ocur, int *next, unsigned int *next_count, unsigned int *defer_count)
 {
     const int lane = lane_id();
     const int last = a.n - 1;
@@ -140,6 +140,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, int lds_
 
     if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
         if (lane == 0) {
+            a.level[k] = round;
             a.out_start[k] = 0;
             a.out_cnt[k] = 0;
             a.diag[k] = 0.0;
@@ -302,14 +303,15 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, char *lds, int lds_
         if (lane == 0) raise_status(a.ctr, kStatusOutPool);
         return false;
     }
-    const int lk = a.level[k];
+    // a column is eliminated in the round after its last dependency, so its
+    // level in the triangular-solve DAG is its round
     for (int i = lane; i < d; i += kWave) {
         int r = b.mrow[i];
         a.out_row[obase + i] = r;
         a.out_val[obase + i] = -(b.mw[i] / S);
-        if (r != last) atomicMax(&a.level[r], lk + 1);
     }
     if (lane == 0) {
+        a.level[k] = round;
         a.out_start[k] = obase;
         a.out_cnt[k] = d;
         a.diag[k] = S;
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
     bool ok = true;
     for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
         int col = cur[i];
-        if (!eliminate_column(a, col, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
+        if (!eliminate_column(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
                               &a.big_count[round])) { ok = false; break; }
     }
     (void)ok;
@@ -382,7 +384,8 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int col = a.big_list[i];
-        if (!eliminate_column(a, col, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr)) break;
+        if (!eliminate_column(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
+            break;
     }
     if (lane_id() == 0) {
         ws[0] = fcur.cur; ws[1] = fcur.end;
