@@ -99,6 +99,44 @@ __device__ void wave_sort(int *row, double *w, int *pay, int len)
     }
 }
 
+// The same ordering for len <= 64 entries held one per lane: a bitonic network
+// over lane pairs (shuffles), no LDS round trips between stages.  Positions
+// >= len hold +infinity keys.  Ties keep their values (sorted output is
+// unique: equal keys carry equal payloads in both uses).
+template <bool kByRow>
+__device__ void lane_sort(int *row, double *w, int *pay, int len)
+{
+    const int lane = lane_id();
+    const int P = next_pow2(len);
+    int r = lane < len ? row[lane] : 0x7fffffff;
+    double x = lane < len ? w[lane] : __builtin_inf();
+    int y = (lane < len && pay) ? pay[lane] : 0;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int ro = __shfl_xor(r, j, kWave);
+            const double xo = __shfl_xor(x, j, kWave);
+            const int yo = __shfl_xor(y, j, kWave);
+            const bool o_less = kByRow ? (ro < r || (ro == r && xo < x)) : (xo < x || (xo == x && ro < r));
+            const bool o_more = kByRow ? (ro > r || (ro == r && xo > x)) : (xo > x || (xo == x && ro > r));
+            const bool keep_min = ((lane & k) == 0) == ((lane & j) == 0);
+            if (keep_min ? o_less : o_more) { r = ro; x = xo; y = yo; }
+        }
+    }
+    if (lane < len) {
+        row[lane] = r;
+        w[lane] = x;
+        if (pay) pay[lane] = y;
+    }
+    __syncthreads();
+}
+
+template <bool kByRow>
+__device__ __forceinline__ void column_sort(int *row, double *w, int *pay, int len)
+{
+    if (len <= kWave) lane_sort<kByRow>(row, w, pay, len);
+    else wave_sort<kByRow>(row, w, pay, len);
+}
+
 __device__ __forceinline__ void raise_status(unsigned long long *ctr, unsigned long long bits)
 {
     atomicOr(&ctr[kCtrStatus], bits);
@@ -205,7 +243,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     __syncthreads();
 
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
-    wave_sort<true>(b.srow, b.sw, b.smult, m);
+    column_sort<true>(b.srow, b.sw, b.smult, m);
     int d = 0;
     for (int base_i = 0; base_i < m; base_i += kWave) {
         int i = base_i + lane;
@@ -244,7 +282,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
         b.smult[t] = t;
     }
     __syncthreads();
-    wave_sort<false>(b.srow, b.sw, b.smult, d);
+    column_sort<false>(b.srow, b.sw, b.smult, d);
     // canonical blocked scan (shared with the oracle): per 64-entry block the
     // doubling scan x_i += x_{i-off}, off = 1..32, then the previous block's
     // last sum is added
