@@ -272,6 +272,7 @@ struct SolveArgs {
     long long spin_ticks;
     const double *b;
     int zero;                     // always 0 (keeps some loads per-lane, see k_sweep_bulk)
+    int nap_max;                  // longest back-off between polls of a pending value (x s_sleep 2)
     // batched systems: system of every position and the per-system alpha
     // used by the forward sweep's r/x update (both null for one system)
     const int *sys_of;

@@ -70,7 +70,7 @@ __device__ __forceinline__ bool last_block(unsigned int *arrive, int *lds_flag)
 }
 
 template <typename T>
-__device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
+__device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc, int nap_max)
 {
     long long t0 = wall_clock64();
     int nap = 1;
@@ -84,7 +84,7 @@ __device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
         }
         // exponential back-off keeps waiting waves from flooding memory
         for (int q = 0; q < nap; ++q) __builtin_amdgcn_s_sleep(2);
-        nap = nap < 32 ? nap * 2 : 32;
+        nap = nap < nap_max ? nap * 2 : nap_max;
     }
 }
 
@@ -93,7 +93,7 @@ __device__ __noinline__ T wait_slow(T *p, long long spin_ticks, PcgScalars *sc)
 // are issued eight at a time; a value still holding the sentinel is polled.
 template <typename T>
 __device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int nsl, int stride, T *vec, T acc,
-                                               long long spin, PcgScalars *sc)
+                                               long long spin, PcgScalars *sc, int nap_max)
 {
     constexpr int B = 8;
     int sl = 0;
@@ -109,7 +109,7 @@ __device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int
         for (int u = 0; u < B; ++u) v[u] = vec[j[u]];
 #pragma unroll
         for (int u = 0; u < B; ++u) {
-            if (is_sent(v[u])) v[u] = wait_slow(vec + j[u], spin, sc);
+            if (is_sent(v[u])) v[u] = wait_slow(vec + j[u], spin, sc, nap_max);
             acc = fma_t(-l[u], v[u], acc);
         }
     }
@@ -117,7 +117,7 @@ __device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int
         int j = ind[sl * stride];
         T l = val[sl * stride];
         T v = vec[j];
-        if (is_sent(v)) v = wait_slow(vec + j, spin, sc);
+        if (is_sent(v)) v = wait_slow(vec + j, spin, sc, nap_max);
         acc = fma_t(-l, v, acc);
     }
     return acc;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, dou
     }
     if (t < stride) {
         const int off = m.z + t;
-        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, m.w, stride, y, acc, a.spin_ticks, a.sc);
+        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, m.w, stride, y, acc, a.spin_ticks, a.sc, a.nap_max);
     }
     if (lead) {
         if (update_rx) {
@@ -294,7 +294,7 @@ __device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
     }
     if (t < stride) {
         const int off = m.z + t;
-        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, m.w, stride, z, acc, a.spin_ticks, a.sc);
+        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, m.w, stride, z, acc, a.spin_ticks, a.sc, a.nap_max);
     }
     if (lead) {
         y[k] = sent_value(T());

@@ -108,6 +108,7 @@ struct rc_handle_s {
     int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
+    int nap_max = 32;
     int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
     DevVec tl_start, tl_cnt, tp_ptr, tp_ind, tp_val, tf_off, tf_slices, tf_ind, tf_val, tb_off, tb_slices, tb_ind,
         tb_val, ti_ptr, ti_ind, ti_val, tacc;
@@ -255,6 +256,8 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
         if (lo && std::string(lo) == "length") h->level_order = 0;
         if (lo && std::string(lo) == "spatial") h->level_order = 1;
         if (lo && std::string(lo) == "bucket") h->level_order = 2;
+        const char *nm = std::getenv("RCHOL_NAP_MAX");          // polling back-off cap (1..64)
+        if (nm) h->nap_max = std::max(1, std::min(64, std::atoi(nm)));
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
         if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
     }
@@ -1475,6 +1478,7 @@ static SolveArgs solve_args(rc_handle_t h)
     a.spin_ticks = h->spin_ticks;
     a.b = h->b.as<double>();
     a.zero = 0;
+    a.nap_max = h->nap_max;
     a.sys_of = nullptr;
     a.alpha_sys = nullptr;
     if (h->nsys > 0) {
