@@ -457,25 +457,24 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     int4 *tm = reinterpret_cast<int4 *>(reinterpret_cast<char *>(tail_lds_d) + tail_value_bytes<T>(nt));
     for (int L = t; L < ntl; L += kTailThreads) tm[L] = make_int4(a.tl_start[L], a.tl_cnt[L], off[L], slices[L]);
     __syncthreads();
-    // Software pipeline over two register sets (A, B): this lane's first
-    // kTailPref index/value slices of level s + 1 are loaded while level s is
-    // solved; the loop is unrolled by two so that no register copy (and no
-    // wait) sits between the loads and their use.
-    // Loads are issued unconditionally (a lane without a slice reads entry 0
-    // and discards it), so every path has the same number of loads in flight
-    // and the compiler's waits stay exact.
+    // Software pipeline over three register sets (A, B, C): this lane's first
+    // kTailPref index/value slices of level s + 2 are loaded while level s is
+    // solved (two levels of latency cover one global round trip); the loop
+    // is unrolled by three so that no register copy (and no wait) sits
+    // between the loads and their use.
     auto fetch = [&](int s, int4 &d, int *j, T *l) {
         s = s < ntl ? s : ntl - 1;
         d = tm[kFwd ? s : ntl - 1 - s];
         const int stride = sweep_rows_pow2(d.y) * tail_group(d.y);
 #pragma unroll
         for (int u = 0; u < kTailPref; ++u) {
+            // every lane loads (entry 0 when it has no slice u; its index is a
+            // valid LDS slot) and the value is masked arithmetically, so the
+            // loads are never predicated away
             const bool has = t < stride && u < d.w;
             const int e = has ? d.z + t + u * stride : 0;
-            const int jj = ind[e];
-            const T ll = val[e];
-            j[u] = has ? jj : nt;
-            l[u] = has ? ll : (T)0;
+            j[u] = ind[e];
+            l[u] = val[e] * (has ? (T)1 : (T)0);
         }
     };
     auto solve = [&](const int4 &d, const int *j, const T *l) {
@@ -496,16 +495,22 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         if (lead) v[q] = acc;
         lds_barrier();
     };
-    int4 dA = make_int4(0, 0, 0, 0), dB = make_int4(0, 0, 0, 0);
-    int jA[kTailPref], jB[kTailPref];
-    T lA[kTailPref], lB[kTailPref];
-    if (ntl > 0) fetch(0, dA, jA, lA);
-    for (int s = 0; s < ntl; s += 2) {
-        fetch(s + 1, dB, jB, lB);
+    int4 dA = make_int4(0, 0, 0, 0), dB = dA, dC = dA;
+    int jA[kTailPref], jB[kTailPref], jC[kTailPref];
+    T lA[kTailPref], lB[kTailPref], lC[kTailPref];
+    if (ntl > 0) {
+        fetch(0, dA, jA, lA);
+        fetch(1, dB, jB, lB);
+    }
+    for (int s = 0; s < ntl; s += 3) {
+        fetch(s + 2, dC, jC, lC);
         solve(dA, jA, lA);
         if (s + 1 >= ntl) break;
-        fetch(s + 2, dA, jA, lA);
+        fetch(s + 3, dA, jA, lA);
         solve(dB, jB, lB);
+        if (s + 2 >= ntl) break;
+        fetch(s + 4, dB, jB, lB);
+        solve(dC, jC, lC);
     }
     for (int q = t; q < nt; q += kTailThreads) {
         const int k = tp0 + q;

@@ -108,7 +108,7 @@ struct rc_handle_s {
     int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
-    int nap_max = 32;
+    int nap_max = 1;                // measured best: poll again after one s_sleep 2
     int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
     DevVec tl_start, tl_cnt, tp_ptr, tp_ind, tp_val, tf_off, tf_slices, tf_ind, tf_val, tb_off, tb_slices, tb_ind,
         tb_val, ti_ptr, ti_ind, ti_val, tacc;
