@@ -99,41 +99,87 @@ __device__ void wave_sort(int *row, double *w, int *pay, int len)
     }
 }
 
-// The same ordering for len <= 64 entries held one per lane: a bitonic network
-// over lane pairs (shuffles), no LDS round trips between stages.  Positions
-// >= len hold +infinity keys.  Ties keep their values (sorted output is
-// unique: equal keys carry equal payloads in both uses).
+// The same ordering for len <= 64 R entries held in registers, entry i in
+// slot i / 64 of lane i % 64: a bitonic network whose comparator pairs are
+// lane pairs (shuffles) for distances below 64 and slot pairs inside a lane
+// above, with no LDS round trips between stages.  Positions >= len hold
+// +infinity keys.  Ties keep their values (the sorted output is unique: equal
+// keys carry equal payloads in both uses).
 template <bool kByRow>
+__device__ __forceinline__ bool key_less(int ra, double wa, int rb, double wb)
+{
+    return kByRow ? (ra < rb || (ra == rb && wa < wb)) : (wa < wb || (wa == wb && ra < rb));
+}
+
+template <bool kByRow, int R>
 __device__ void lane_sort(int *row, double *w, int *pay, int len)
 {
     const int lane = lane_id();
     const int P = next_pow2(len);
-    int r = lane < len ? row[lane] : 0x7fffffff;
-    double x = lane < len ? w[lane] : __builtin_inf();
-    int y = (lane < len && pay) ? pay[lane] : 0;
+    int r[R], y[R];
+    double x[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int i = s * kWave + lane;
+        r[s] = i < len ? row[i] : 0x7fffffff;
+        x[s] = i < len ? w[i] : __builtin_inf();
+        y[s] = (i < len && pay) ? pay[i] : 0;
+    }
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const int ro = __shfl_xor(r, j, kWave);
-            const double xo = __shfl_xor(x, j, kWave);
-            const int yo = __shfl_xor(y, j, kWave);
-            const bool o_less = kByRow ? (ro < r || (ro == r && xo < x)) : (xo < x || (xo == x && ro < r));
-            const bool o_more = kByRow ? (ro > r || (ro == r && xo > x)) : (xo > x || (xo == x && ro > r));
-            const bool keep_min = ((lane & k) == 0) == ((lane & j) == 0);
-            if (keep_min ? o_less : o_more) { r = ro; x = xo; y = yo; }
+            if (j >= kWave) {
+                const int js = j / kWave;
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int t = s ^ js;
+                    if (t > s && t < R) {
+                        const bool asc = ((s * kWave + lane) & k) == 0;
+                        const bool swap = asc ? key_less<kByRow>(r[t], x[t], r[s], x[s])
+                                              : key_less<kByRow>(r[s], x[s], r[t], x[t]);
+                        if (swap) {
+                            const int tr = r[s]; r[s] = r[t]; r[t] = tr;
+                            const double tx = x[s]; x[s] = x[t]; x[t] = tx;
+                            const int ty = y[s]; y[s] = y[t]; y[t] = ty;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    const int ro = __shfl_xor(r[s], j, kWave);
+                    const double xo = __shfl_xor(x[s], j, kWave);
+                    const int yo = __shfl_xor(y[s], j, kWave);
+                    const bool asc = ((s * kWave + lane) & k) == 0;
+                    const bool keep_min = asc == ((lane & j) == 0);
+                    const bool take = keep_min ? key_less<kByRow>(ro, xo, r[s], x[s])
+                                               : key_less<kByRow>(r[s], x[s], ro, xo);
+                    if (take) { r[s] = ro; x[s] = xo; y[s] = yo; }
+                }
+            }
         }
     }
-    if (lane < len) {
-        row[lane] = r;
-        w[lane] = x;
-        if (pay) pay[lane] = y;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int i = s * kWave + lane;
+        if (i < len) {
+            row[i] = r[s];
+            w[i] = x[s];
+            if (pay) pay[i] = y[s];
+        }
     }
     __syncthreads();
 }
 
-template <bool kByRow>
+// kMaxR bounds the register slots (and so the kernel's register budget): the
+// round kernel only sees columns of <= 128 entries, the deferred-column kernel
+// sorts up to 512 in registers.
+template <bool kByRow, int kMaxR>
 __device__ __forceinline__ void column_sort(int *row, double *w, int *pay, int len)
 {
-    if (len <= kWave) lane_sort<kByRow>(row, w, pay, len);
+    if (len <= kWave) lane_sort<kByRow, 1>(row, w, pay, len);
+    else if (kMaxR >= 2 && len <= 2 * kWave) lane_sort<kByRow, (kMaxR >= 2 ? 2 : 1)>(row, w, pay, len);
+    else if (kMaxR >= 4 && len <= 4 * kWave) lane_sort<kByRow, (kMaxR >= 4 ? 4 : 1)>(row, w, pay, len);
+    else if (kMaxR >= 8 && len <= 8 * kWave) lane_sort<kByRow, (kMaxR >= 8 ? 8 : 1)>(row, w, pay, len);
     else wave_sort<kByRow>(row, w, pay, len);
 }
 
@@ -166,6 +212,7 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
 
 // Eliminates column k; newly ready neighbours are appended to `next`.
 // Returns false if the pools or the scratch are exhausted (status set).
+template <int kMaxR>
 __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
                                  Cursor &ocur, int *next, unsigned int *next_count, unsigned int *defer_count)
 {
@@ -243,7 +290,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     __syncthreads();
 
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
-    column_sort<true>(b.srow, b.sw, b.smult, m);
+    column_sort<true, kMaxR>(b.srow, b.sw, b.smult, m);
     int d = 0;
     for (int base_i = 0; base_i < m; base_i += kWave) {
         int i = base_i + lane;
@@ -282,7 +329,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
         b.smult[t] = t;
     }
     __syncthreads();
-    column_sort<false>(b.srow, b.sw, b.smult, d);
+    column_sort<false, kMaxR>(b.srow, b.sw, b.smult, d);
     // canonical blocked scan (shared with the oracle): per 64-entry block the
     // doubling scan x_i += x_{i-off}, off = 1..32, then the previous block's
     // last sum is added
@@ -397,7 +444,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
     bool ok = true;
     for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
         int col = cur[i];
-        if (!eliminate_column(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
+        if (!eliminate_column<2>(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
                               &a.big_count[round])) { ok = false; break; }
     }
     (void)ok;
@@ -422,7 +469,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int col = a.big_list[i];
-        if (!eliminate_column(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
+        if (!eliminate_column<8>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
             break;
     }
     if (lane_id() == 0) {
