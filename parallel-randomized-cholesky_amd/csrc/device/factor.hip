@@ -170,9 +170,9 @@ __device__ void lane_sort(int *row, double *w, int *pay, int len)
     __syncthreads();
 }
 
-// kMaxR bounds the register slots (and so the kernel's register budget): the
-// round kernel only sees columns of <= 128 entries, the deferred-column kernel
-// sorts up to 512 in registers.
+// kMaxR bounds the register slots (and so the kernel's register budget).  Both
+// kernels use 2: above 128 entries the LDS network measured faster than
+// 4 or 8 slots per lane.
 template <bool kByRow, int kMaxR>
 __device__ __forceinline__ void column_sort(int *row, double *w, int *pay, int len)
 {
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int col = a.big_list[i];
-        if (!eliminate_column<8>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
+        if (!eliminate_column<2>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
             break;
     }
     if (lane_id() == 0) {
