@@ -210,6 +210,40 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
     return true;
 }
 
+// Gathers column k's strict-lower entries and its pending fills (kFillLists
+// linked lists, walked in lockstep, one lane per list) into b, writing the
+// first `cap` entries; returns the total count.  The order of the gathered
+// entries does not matter: the (row, weight) sort makes it canonical.
+__device__ int gather_column(const FactorArgs &a, int k, int lo0, int norig, const ColBuf &b, int cap)
+{
+    const int lane = lane_id();
+    for (int t = lane; t < norig && t < cap; t += kWave) {
+        b.srow[t] = a.lo_row[lo0 + t];
+        b.sw[t] = a.lo_w[lo0 + t];
+        b.smult[t] = 1;
+    }
+    int f = lane < kFillLists ? a.head[(size_t)k * kFillLists + lane] : -1;
+    int t = norig;
+    while (true) {
+        const bool act = f >= 0;
+        const uint64_t mask = ballot(act);
+        if (!mask) break;
+        if (act) {
+            const FillRec rec = a.pool[f];
+            const int pos = t + lanes_below(mask);
+            if (pos < cap) {
+                b.srow[pos] = rec.row;
+                b.sw[pos] = rec.w;
+                b.smult[pos] = 1;
+            }
+            f = rec.next;
+        }
+        t += __popcll(mask);
+    }
+    __syncthreads();
+    return t;
+}
+
 // Eliminates column k; newly ready neighbours are appended to `next`.
 // Returns false if the pools or the scratch are exhausted (status set).
 template <int kMaxR>
@@ -220,9 +254,13 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     const int last = a.n - 1;
     const int lo0 = a.lo_ptr[k];
     const int norig = a.lo_ptr[k + 1] - lo0;
-    const int nf_in = a.nfill[k];
-    const int m = norig + nf_in;
 
+    // ---- 1. gather (driver_local.cpp:459-487; cpu auxilliary.hpp:89-173)
+    // into this kernel's LDS; the entries are counted as they come (no fill
+    // counter is kept), and a column that turns out larger than the buffer is
+    // deferred (round kernel) or gathered again into the global scratch
+    ColBuf b = carve(lds, lds_cap);
+    int m = gather_column(a, k, lo0, norig, b, lds_cap);
     if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
         if (lane == 0) {
             a.level[k] = round;
@@ -232,62 +270,26 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
         }
         return true;
     }
-
-    // ---- choose the working buffer
     const int P = next_pow2(m);
-    char *base;
-    int cap;
-    if (P > lds_cap && defer_count) {
-        // too big for this kernel's LDS: the round's large-LDS kernel takes it
-        if (lane == 0) a.big_list[atomicAdd(defer_count, 1u)] = k;
-        return true;
-    }
-    if (P <= lds_cap) {
-        base = lds;
-        cap = lds_cap;
-    } else if (P <= a.scratch_cap) {
-        base = a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry;
-        cap = a.scratch_cap;
-    } else {
-        if (lane == 0) {
-            atomicMax(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
-            raise_status(a.ctr, kStatusScratch);
+    if (P > lds_cap) {
+        if (defer_count) {
+            // too big for this kernel's LDS: the round's large-LDS kernel takes it
+            if (lane == 0) a.big_list[atomicAdd(defer_count, 1u)] = k;
+            return true;
         }
-        return false;
-    }
-    ColBuf b = carve(base, cap);
-
-    // ---- 1. gather (driver_local.cpp:459-487; cpu auxilliary.hpp:89-173)
-    for (int t = lane; t < norig; t += kWave) {
-        b.srow[t] = a.lo_row[lo0 + t];
-        b.sw[t] = a.lo_w[lo0 + t];
-        b.smult[t] = 1;
-    }
-    // pending fills: kFillLists lists per column, walked in lockstep (one
-    // lane per list); the order of the gathered entries does not matter, the
-    // (row, weight) sort below makes it canonical
-    {
-        int f = lane < kFillLists ? a.head[(size_t)k * kFillLists + lane] : -1;
-        int t = 0;
-        while (true) {
-            const bool act = f >= 0;
-            const uint64_t mask = ballot(act);
-            if (!mask) break;
-            if (act) {
-                const FillRec rec = a.pool[f];
-                const int pos = t + lanes_below(mask);
-                if (pos < nf_in) {
-                    b.srow[norig + pos] = rec.row;
-                    b.sw[norig + pos] = rec.w;
-                    b.smult[norig + pos] = 1;
-                }
-                f = rec.next;
+        if (P > a.scratch_cap) {
+            if (lane == 0) {
+                atomicMax(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
+                raise_status(a.ctr, kStatusScratch);
             }
-            t += __popcll(mask);
+            return false;
         }
-        if (lane == 0 && t != nf_in) raise_status(a.ctr, kStatusInconsistent);
+        b = carve(a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry, a.scratch_cap);
+        if (gather_column(a, k, lo0, norig, b, a.scratch_cap) != m) {
+            if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
+            return false;
+        }
     }
-    __syncthreads();
 
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
     column_sort<true, kMaxR>(b.srow, b.sw, b.smult, m);
@@ -378,7 +380,6 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
         fr.next = old;
         fr.w = fw;
         a.pool[rec] = fr;
-        atomicAdd(&a.nfill[cc], 1);
         atomicAdd(&b.minc[ri > rj ? b.smult[i] : b.smult[lo]], 1);   // rr gains a dependency
     }
 
@@ -480,7 +481,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
 
 // ---- set-up kernels ------------------------------------------------------
 
-__global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, int *nfill, int *level, int *out_cnt,
+__global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, int *level, int *out_cnt,
                               long long *out_start, double *diag)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)n * kFillLists;
@@ -488,7 +489,6 @@ __global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, i
         head[i] = -1;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         dep[i] = dep_init[i];
-        nfill[i] = 0;
         level[i] = 0;
         out_cnt[i] = 0;
         out_start[i] = 0;
@@ -514,7 +514,7 @@ __global__ void k_factor_seed(int n, const int *dep, int *list, unsigned int *co
 
 void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_factor_init, dim3(1024), dim3(256), 0, s, a.n, dep_init, a.dep, a.head, a.nfill, a.level,
+    hipLaunchKernelGGL(k_factor_init, dim3(1024), dim3(256), 0, s, a.n, dep_init, a.dep, a.head, a.level,
                        a.out_cnt, a.out_start, a.diag);
     hipLaunchKernelGGL(k_factor_seed, dim3(1024), dim3(256), 0, s, a.n, (const int *)a.dep, a.list0, a.round_count);
 }

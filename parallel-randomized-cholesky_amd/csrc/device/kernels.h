@@ -48,7 +48,7 @@ struct FactorArgs {
     const int *lo_ptr;      // n+1, strict-lower part of the Laplacian (CSC)
     const int *lo_row;
     const double *lo_w;     // |A[r,k]|
-    int *dep, *head, *nfill, *level;   // head: n * kFillLists
+    int *dep, *head, *level;   // head: n * kFillLists
     int *list0, *list1;     // ready lists of even / odd rounds
     unsigned int *round_count;   // [round] = columns eliminated in that round
     long long *wave_state;  // per workgroup: fill cursor, fill end, out cursor, out end

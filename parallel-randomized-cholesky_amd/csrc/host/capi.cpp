@@ -87,7 +87,7 @@ struct rc_handle_s {
     // factor inputs / working set
     DevVec lo_ptr, lo_row, lo_w, dep_init;
     DevVec big_list, big_count;
-    DevVec dep, head, nfill, level, list0, list1, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
+    DevVec dep, head, level, list0, list1, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
         out_cnt, diag64, scratch;
     long long pool_cap = 0, out_cap = 0;
     int scratch_cap = 0;
@@ -723,7 +723,6 @@ static int alloc_factor_work(rc_handle_t h)
     const size_t nf = (size_t)h->nf;
     RC_HIP(h->dep.ensure(nf * sizeof(int)));
     RC_HIP(h->head.ensure(nf * kFillLists * sizeof(int)));
-    RC_HIP(h->nfill.ensure(nf * sizeof(int)));
     RC_HIP(h->level.ensure(nf * sizeof(int)));
     RC_HIP(h->list0.ensure(nf * sizeof(int)));
     RC_HIP(h->list1.ensure(nf * sizeof(int)));
@@ -752,7 +751,6 @@ static FactorArgs factor_args(rc_handle_t h)
     a.lo_w = h->lo_w.as<double>();
     a.dep = h->dep.as<int>();
     a.head = h->head.as<int>();
-    a.nfill = h->nfill.as<int>();
     a.level = h->level.as<int>();
     a.list0 = h->list0.as<int>();
     a.list1 = h->list1.as<int>();
