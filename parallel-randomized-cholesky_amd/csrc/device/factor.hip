@@ -38,13 +38,13 @@ namespace rc {
 namespace {
 
 constexpr int kLdsCap = 128;                                   // entries per wave in LDS (round kernel)
-constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc
+constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc jt
 constexpr long long kFillChunk = 2048;
 constexpr long long kOutChunk = 2048;
 
 struct ColBuf {
     double *sw, *mw, *C;
-    int *srow, *smult, *mrow, *mmult, *minc;
+    int *srow, *smult, *mrow, *mmult, *minc, *jt;   // jt: sampled partner of every fill
 };
 
 __device__ __forceinline__ ColBuf carve(char *base, int cap)
@@ -58,6 +58,7 @@ __device__ __forceinline__ ColBuf carve(char *base, int cap)
     b.mrow = b.smult + cap;
     b.mmult = b.mrow + cap;
     b.minc = b.mmult + cap;
+    b.jt = b.minc + cap;
     return b;
 }
 
@@ -73,7 +74,7 @@ __device__ __forceinline__ int next_pow2(int v)
 // positions >= len behave as +infinity and need no padding.
 // kByRow: key (row, w); otherwise key (w, row).  pay is carried if non-null.
 template <bool kByRow>
-__device__ void wave_sort(int *row, double *w, int *pay, int len)
+__device__ __forceinline__ void wave_sort(int *row, double *w, int *pay, int len)
 {
     const int lane = lane_id();
     const int P = next_pow2(len);
@@ -112,7 +113,7 @@ __device__ __forceinline__ bool key_less(int ra, double wa, int rb, double wb)
 }
 
 template <bool kByRow, int R>
-__device__ void lane_sort(int *row, double *w, int *pay, int len)
+__device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len)
 {
     const int lane = lane_id();
     const int P = next_pow2(len);
@@ -211,10 +212,11 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
 }
 
 // Gathers column k's strict-lower entries and its pending fills (kFillLists
-// linked lists, walked in lockstep, one lane per list) into b, writing the
-// first `cap` entries; returns the total count.  The order of the gathered
-// entries does not matter: the (row, weight) sort makes it canonical.
-__device__ int gather_column(const FactorArgs &a, int k, int lo0, int norig, const ColBuf &b, int cap)
+// linked lists, walked in lockstep, one lane per list; f = this lane's list
+// head) into b, writing the first `cap` entries; returns the total count.
+// The order of the gathered entries does not matter: the (row, weight) sort
+// makes it canonical.
+__device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int norig, int f, const ColBuf &b, int cap)
 {
     const int lane = lane_id();
     for (int t = lane; t < norig && t < cap; t += kWave) {
@@ -222,7 +224,6 @@ __device__ int gather_column(const FactorArgs &a, int k, int lo0, int norig, con
         b.sw[t] = a.lo_w[lo0 + t];
         b.smult[t] = 1;
     }
-    int f = lane < kFillLists ? a.head[(size_t)k * kFillLists + lane] : -1;
     int t = norig;
     while (true) {
         const bool act = f >= 0;
@@ -244,53 +245,15 @@ __device__ int gather_column(const FactorArgs &a, int k, int lo0, int norig, con
     return t;
 }
 
-// Eliminates column k; newly ready neighbours are appended to `next`.
-// Returns false if the pools or the scratch are exhausted (status set).
+// Steps 2-5 of a column whose m >= 1 entries are gathered in b.  Inlined at
+// every call so that b's accesses compile to LDS or global instructions (a
+// generic pointer would make each LDS access wait for all memory traffic).
 template <int kMaxR>
-__device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
-                                 Cursor &ocur, int *next, unsigned int *next_count, unsigned int *defer_count)
+__device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int round, const ColBuf &b, int m,
+                                               Cursor &fcur, Cursor &ocur, int *next, unsigned int *next_count)
 {
     const int lane = lane_id();
     const int last = a.n - 1;
-    const int lo0 = a.lo_ptr[k];
-    const int norig = a.lo_ptr[k + 1] - lo0;
-
-    // ---- 1. gather (driver_local.cpp:459-487; cpu auxilliary.hpp:89-173)
-    // into this kernel's LDS; the entries are counted as they come (no fill
-    // counter is kept), and a column that turns out larger than the buffer is
-    // deferred (round kernel) or gathered again into the global scratch
-    ColBuf b = carve(lds, lds_cap);
-    int m = gather_column(a, k, lo0, norig, b, lds_cap);
-    if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
-        if (lane == 0) {
-            a.level[k] = round;
-            a.out_start[k] = 0;
-            a.out_cnt[k] = 0;
-            a.diag[k] = 0.0;
-        }
-        return true;
-    }
-    const int P = next_pow2(m);
-    if (P > lds_cap) {
-        if (defer_count) {
-            // too big for this kernel's LDS: the round's large-LDS kernel takes it
-            if (lane == 0) a.big_list[atomicAdd(defer_count, 1u)] = k;
-            return true;
-        }
-        if (P > a.scratch_cap) {
-            if (lane == 0) {
-                atomicMax(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
-                raise_status(a.ctr, kStatusScratch);
-            }
-            return false;
-        }
-        b = carve(a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry, a.scratch_cap);
-        if (gather_column(a, k, lo0, norig, b, a.scratch_cap) != m) {
-            if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
-            return false;
-        }
-    }
-
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
     column_sort<true, kMaxR>(b.srow, b.sw, b.smult, m);
     int d = 0;
@@ -352,13 +315,9 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     __syncthreads();
     const double S = b.C[d - 1];
 
-    // ---- 4. sampled fills (:598-631; gpu auxilliary.hpp:543-592)
+    // ---- 4. sampled partners (:598-631; gpu auxilliary.hpp:543-592):
+    //         partner j of every fill i and the new dependencies per neighbour
     const int nfill_out = d - 1;
-    long long fbase = 0;
-    if (nfill_out > 0 && !carve_pool(fcur, nfill_out, kFillChunk, &a.ctr[kCtrFillTop], a.pool_cap, fbase)) {
-        if (lane == 0) raise_status(a.ctr, kStatusFillPool);
-        return false;
-    }
     for (int i = lane; i < nfill_out; i += kWave) {
         double u = sample_uniform(a.seed, (uint32_t)i, (uint32_t)k);
         double Ci = b.C[i];
@@ -369,55 +328,62 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
             int mid = lo + ((hi - lo) >> 1);
             if (b.C[mid] - Ci < tgt) lo = mid + 1; else hi = mid;
         }
-        int ri = b.srow[i], rj = b.srow[lo];
-        int rr = ri > rj ? ri : rj;
-        int cc = ri > rj ? rj : ri;
-        double fw = (b.sw[i] * rest) / S;
-        int rec = (int)(fbase + i);
-        int old = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
-        FillRec fr;
-        fr.row = rr;
-        fr.next = old;
-        fr.w = fw;
-        a.pool[rec] = fr;
-        atomicAdd(&b.minc[ri > rj ? b.smult[i] : b.smult[lo]], 1);   // rr gains a dependency
+        b.jt[i] = lo;
+        const int ri = b.srow[i], rj = b.srow[lo];
+        atomicAdd(&b.minc[ri > rj ? b.smult[i] : b.smult[lo]], 1);   // the larger row gains a dependency
     }
-
-    // ---- 5. factor column (row order, -w/S), pivot S, levels (:638-645)
-    long long obase = 0;
+    long long fbase = 0, obase = 0;
+    if (nfill_out > 0 && !carve_pool(fcur, nfill_out, kFillChunk, &a.ctr[kCtrFillTop], a.pool_cap, fbase)) {
+        if (lane == 0) raise_status(a.ctr, kStatusFillPool);
+        return false;
+    }
     if (!carve_pool(ocur, d, kOutChunk, &a.ctr[kCtrOutTop], a.out_cap, obase)) {
         if (lane == 0) raise_status(a.ctr, kStatusOutPool);
         return false;
     }
     // a column is eliminated in the round after its last dependency, so its
-    // level in the triangular-solve DAG is its round
-    for (int i = lane; i < d; i += kWave) {
-        int r = b.mrow[i];
-        a.out_row[obase + i] = r;
-        a.out_val[obase + i] = -(b.mw[i] / S);
-    }
+    // level in the triangular-solve DAG is its round (:638-645)
     if (lane == 0) {
         a.level[k] = round;
         a.out_start[k] = obase;
         a.out_cnt[k] = d;
         a.diag[k] = S;
     }
-    __syncthreads();   // minc complete
+    __syncthreads();   // minc and jt complete
 
-    // ---- 6. retire: one net dependency update per neighbour (:653-687)
+    // ---- 5. per 64 entries: the neighbour's net dependency update (:653-687),
+    //         the fill edge (max, min, w_i (S - C_i) / S) pushed onto column
+    //         min's lists, and the factor entry -w/S in row order; the
+    //         dependency add and the list push are in flight together
     for (int base_i = 0; base_i < d; base_i += kWave) {
-        int i = base_i + lane;
-        bool ready = false;
-        int r = -1;
-        if (i < d) {
-            r = b.mrow[i];
-            int delta = b.minc[i] - b.mmult[i];
-            if (r != last && delta != 0) {
-                int old = atomicAdd(&a.dep[r], delta);
-                ready = (old + delta == 0);
-            }
+        const int e = base_i + lane;
+        bool upd = false;
+        int r = -1, delta = 0, old_dep = 0;
+        if (e < d) {
+            r = b.mrow[e];
+            delta = b.minc[e] - b.mmult[e];
+            upd = r != last && delta != 0;
+            if (upd) old_dep = atomicAdd(&a.dep[r], delta);
+            a.out_row[obase + e] = r;
+            a.out_val[obase + e] = -(b.mw[e] / S);
         }
-        uint64_t mask = ballot(ready);
+        if (e < nfill_out) {
+            const int j = b.jt[e];
+            const int ri = b.srow[e], rj = b.srow[j];
+            const int rr = ri > rj ? ri : rj;
+            const int cc = ri > rj ? rj : ri;
+            const double rest = S - b.C[e];
+            const double fw = (b.sw[e] * rest) / S;
+            const int rec = (int)(fbase + e);
+            const int old = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
+            FillRec fr;
+            fr.row = rr;
+            fr.next = old;
+            fr.w = fw;
+            a.pool[rec] = fr;
+        }
+        const bool ready = upd && old_dep + delta == 0;
+        const uint64_t mask = ballot(ready);
         if (mask) {
             unsigned int nb = 0;
             if (lane == 0) nb = atomicAdd(next_count, (unsigned int)__popcll(mask));
@@ -429,23 +395,83 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     return true;
 }
 
+// Eliminates column k; newly ready neighbours are appended to `next`.
+// Returns false if the pools or the scratch are exhausted (status set).
+// kDefer (round kernel): a column larger than the LDS buffer is appended to
+// the round's deferred list; otherwise (large-LDS kernel) it is gathered
+// again into this workgroup's global scratch.
+template <int kMaxR, bool kDefer>
+__device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
+                                 Cursor &ocur, int *next, unsigned int *next_count, unsigned int *defer_count)
+{
+    const int lane = lane_id();
+    // the list heads do not depend on the column's slice: one memory trip for both
+    const int f0 = lane < kFillLists ? a.head[(size_t)k * kFillLists + lane] : -1;
+    const int lo0 = a.lo_ptr[k];
+    const int norig = a.lo_ptr[k + 1] - lo0;
+
+    // ---- 1. gather (driver_local.cpp:459-487; cpu auxilliary.hpp:89-173)
+    // into this kernel's LDS; the entries are counted as they come (no fill
+    // counter is kept), and a column that turns out larger than the buffer is
+    // deferred (round kernel) or gathered again into the global scratch
+    const ColBuf b = carve(lds, lds_cap);
+    int m = gather_column(a, lo0, norig, f0, b, lds_cap);
+    if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
+        if (lane == 0) {
+            a.level[k] = round;
+            a.out_start[k] = 0;
+            a.out_cnt[k] = 0;
+            a.diag[k] = 0.0;
+        }
+        return true;
+    }
+    const int P = next_pow2(m);
+    if (P > lds_cap) {
+        if constexpr (kDefer) {
+            // too big for this kernel's LDS: the round's large-LDS kernel takes it
+            if (lane == 0) a.big_list[atomicAdd(defer_count, 1u)] = k;
+            return true;
+        } else {
+            if (P > a.scratch_cap) {
+                if (lane == 0) {
+                    atomicMax(&a.ctr[kCtrScratchNeed], (unsigned long long)P);
+                    raise_status(a.ctr, kStatusScratch);
+                }
+                return false;
+            }
+            const ColBuf g = carve(a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry, a.scratch_cap);
+            if (gather_column(a, lo0, norig, f0, g, a.scratch_cap) != m) {
+                if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
+                return false;
+            }
+            return process_column<kMaxR>(a, k, round, g, m, fcur, ocur, next, next_count);
+        }
+    }
+    return process_column<kMaxR>(a, k, round, b, m, fcur, ocur, next, next_count);
+}
+
 }  // namespace
 
 // One elimination round: every column of `cur` (count counts[round]).
 __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round, const int *cur, int *next)
 {
     __shared__ __attribute__((aligned(16))) char lds[kLdsCap * kBytesPerEntry];
+    // the first list entry is read together with the count (speculatively,
+    // at a clamped index: it is used only when blockIdx.x < count <= n)
     const unsigned int ncur = a.round_count[round];
+    const unsigned long long status = a.ctr[kCtrStatus];
+    const int first = cur[min(blockIdx.x, (unsigned int)a.n - 1u)];
+    asm volatile("" ::"s"(first), "s"(ncur));   // keeps the three loads in one memory trip
     if (ncur == 0) return;
-    if (a.ctr[kCtrStatus] != 0ull) return;
+    if (status != 0ull) return;
     Cursor fcur, ocur;
     long long *ws = a.wave_state + 4 * (size_t)blockIdx.x;
     fcur.cur = ws[0]; fcur.end = ws[1];
     ocur.cur = ws[2]; ocur.end = ws[3];
     bool ok = true;
     for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
-        int col = cur[i];
-        if (!eliminate_column<2>(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
+        const int col = i == blockIdx.x ? first : cur[i];
+        if (!eliminate_column<2, true>(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
                               &a.big_count[round])) { ok = false; break; }
     }
     (void)ok;
@@ -462,15 +488,18 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
 {
     extern __shared__ __attribute__((aligned(16))) char lds_big[];
     const unsigned int nbig = a.big_count[round];
+    const unsigned long long status = a.ctr[kCtrStatus];
+    const int first = a.big_list[min(blockIdx.x, (unsigned int)a.n - 1u)];   // used only below the count
+    asm volatile("" ::"s"(first), "s"(nbig));
     if (nbig == 0) return;
-    if (a.ctr[kCtrStatus] != 0ull) return;
+    if (status != 0ull) return;
     Cursor fcur, ocur;
     long long *ws = a.wave_state + 4 * ((size_t)a.big_ws + blockIdx.x);
     fcur.cur = ws[0]; fcur.end = ws[1];
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
-        const int col = a.big_list[i];
-        if (!eliminate_column<2>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
+        const int col = i == blockIdx.x ? first : a.big_list[i];
+        if (!eliminate_column<2, false>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
             break;
     }
     if (lane_id() == 0) {
