@@ -211,6 +211,59 @@ __device__ __forceinline__ bool carve_pool(Cursor &c, long long count, long long
     return true;
 }
 
+// ---- segmented ready lists ------------------------------------------------
+__device__ __forceinline__ unsigned int *list_set(const FactorArgs &a, int round)
+{
+    return a.seg_count + (size_t)(round % 3) * kListSlots;
+}
+
+// Lane q: effective entry count c of slot q of a list and the inclusive
+// prefix over slots; the list's length is the prefix at lane 63.
+__device__ __forceinline__ void list_counts(const FactorArgs &a, const unsigned int *set, unsigned int &c,
+                                            unsigned int &incl)
+{
+    const int lane = lane_id();
+    const unsigned int v = set[lane];
+    c = lane < kListSegs ? min(v, (unsigned int)a.seg_cap) : v;
+    incl = c;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned int o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+}
+
+// Storage index of entry i (wave-uniform, below the length) of a list.
+__device__ __forceinline__ size_t list_index(const FactorArgs &a, unsigned int c, unsigned int incl, unsigned int i)
+{
+    const int s = __popcll(ballot(incl <= i));          // slots wholly before entry i
+    const unsigned int excl = __shfl(incl - c, s, kWave);
+    return (size_t)s * (size_t)a.seg_cap + (i - excl);
+}
+
+// Appends `value` of every lane with `ready` to the list of counter set `set`:
+// one atomic on this workgroup's segment counter, and one on the overflow
+// counter for entries past the segment's capacity.
+__device__ __forceinline__ void list_append(const FactorArgs &a, unsigned int *set, int *list, bool ready, int value)
+{
+    const uint64_t mask = ballot(ready);
+    if (!mask) return;
+    const unsigned int cnt = (unsigned int)__popcll(mask);
+    const unsigned int cap = (unsigned int)a.seg_cap;
+    const int seg = (int)(blockIdx.x % kListSegs);
+    unsigned int base = 0;
+    if (lane_id() == 0) base = atomicAdd(&set[seg], cnt);
+    base = bcast(base, 0);
+    const unsigned int pos = base + (unsigned int)lanes_below(mask);
+    if (base + cnt > cap) {
+        const unsigned int lo = base > cap ? base : cap;
+        unsigned int ob = 0;
+        if (lane_id() == 0) ob = atomicAdd(&set[kListSegs], base + cnt - lo);
+        ob = bcast(ob, 0);
+        if (ready && pos >= cap) list[(size_t)kListSegs * cap + ob + (pos - lo)] = value;
+    }
+    if (ready && pos < cap) list[(size_t)seg * cap + pos] = value;
+}
+
 // Gathers column k's strict-lower entries and its pending fills (kFillLists
 // linked lists, walked in lockstep, one lane per list; f = this lane's list
 // head) into b, writing the first `cap` entries; returns the total count.
@@ -250,7 +303,7 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
 // generic pointer would make each LDS access wait for all memory traffic).
 template <int kMaxR>
 __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int round, const ColBuf &b, int m,
-                                               Cursor &fcur, Cursor &ocur, int *next, unsigned int *next_count)
+                                               Cursor &fcur, Cursor &ocur, int *next, unsigned int *next_set)
 {
     const int lane = lane_id();
     const int last = a.n - 1;
@@ -382,14 +435,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
             fr.w = fw;
             a.pool[rec] = fr;
         }
-        const bool ready = upd && old_dep + delta == 0;
-        const uint64_t mask = ballot(ready);
-        if (mask) {
-            unsigned int nb = 0;
-            if (lane == 0) nb = atomicAdd(next_count, (unsigned int)__popcll(mask));
-            nb = bcast(nb, 0);
-            if (ready) next[nb + lanes_below(mask)] = r;
-        }
+        list_append(a, next_set, next, upd && old_dep + delta == 0, r);
     }
     __syncthreads();
     return true;
@@ -402,7 +448,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
 // again into this workgroup's global scratch.
 template <int kMaxR, bool kDefer>
 __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
-                                 Cursor &ocur, int *next, unsigned int *next_count, unsigned int *defer_count)
+                                 Cursor &ocur, int *next, unsigned int *next_set, unsigned int *defer_count)
 {
     const int lane = lane_id();
     // the list heads do not depend on the column's slice: one memory trip for both
@@ -444,10 +490,10 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
                 if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
                 return false;
             }
-            return process_column<kMaxR>(a, k, round, g, m, fcur, ocur, next, next_count);
+            return process_column<kMaxR>(a, k, round, g, m, fcur, ocur, next, next_set);
         }
     }
-    return process_column<kMaxR>(a, k, round, b, m, fcur, ocur, next, next_count);
+    return process_column<kMaxR>(a, k, round, b, m, fcur, ocur, next, next_set);
 }
 
 }  // namespace
@@ -456,23 +502,29 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
 __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round, const int *cur, int *next)
 {
     __shared__ __attribute__((aligned(16))) char lds[kLdsCap * kBytesPerEntry];
-    // the first list entry is read together with the count (speculatively,
-    // at a clamped index: it is used only when blockIdx.x < count <= n)
-    const unsigned int ncur = a.round_count[round];
+    // this round's list counters (one per lane) with the status in one trip;
+    // workgroup 0 clears the counter set of round + 2 (last used by round - 1)
+    // and records the round's length for the host
+    unsigned int c, incl;
+    list_counts(a, list_set(a, round), c, incl);
     const unsigned long long status = a.ctr[kCtrStatus];
-    const int first = cur[min(blockIdx.x, (unsigned int)a.n - 1u)];
-    asm volatile("" ::"s"(first), "s"(ncur));   // keeps the three loads in one memory trip
+    const unsigned int ncur = __shfl(incl, kWave - 1, kWave);
+    if (blockIdx.x == 0) {
+        list_set(a, round + 2)[lane_id()] = 0u;
+        if (lane_id() == 0) a.round_count[round] = ncur;
+    }
     if (ncur == 0) return;
     if (status != 0ull) return;
     Cursor fcur, ocur;
     long long *ws = a.wave_state + 4 * (size_t)blockIdx.x;
     fcur.cur = ws[0]; fcur.end = ws[1];
     ocur.cur = ws[2]; ocur.end = ws[3];
+    unsigned int *next_set = list_set(a, round + 1);
     bool ok = true;
     for (unsigned int i = blockIdx.x; i < ncur; i += gridDim.x) {
-        const int col = i == blockIdx.x ? first : cur[i];
-        if (!eliminate_column<2, true>(a, col, round, lds, kLdsCap, fcur, ocur, next, &a.round_count[round + 1],
-                              &a.big_count[round])) { ok = false; break; }
+        const int col = cur[list_index(a, c, incl, i)];
+        if (!eliminate_column<2, true>(a, col, round, lds, kLdsCap, fcur, ocur, next, next_set,
+                                       &a.big_count[round])) { ok = false; break; }
     }
     (void)ok;
     if (lane_id() == 0) {
@@ -499,7 +551,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int col = i == blockIdx.x ? first : a.big_list[i];
-        if (!eliminate_column<2, false>(a, col, round, lds_big, kBigCap, fcur, ocur, next, &a.round_count[round + 1], nullptr))
+        if (!eliminate_column<2, false>(a, col, round, lds_big, kBigCap, fcur, ocur, next, list_set(a, round + 1), nullptr))
             break;
     }
     if (lane_id() == 0) {
@@ -526,18 +578,11 @@ __global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, i
 }
 
 // Round 0: every column without lower-index neighbours, except the last.
-__global__ void k_factor_seed(int n, const int *dep, int *list, unsigned int *count)
+__global__ void k_factor_seed(FactorArgs a, const int *dep)
 {
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        int i = base + threadIdx.x;
-        bool ready = i < n - 1 && dep[i] == 0;
-        uint64_t mask = ballot(ready);
-        if (mask) {
-            unsigned int nb = 0;
-            if (lane_id() == 0) nb = atomicAdd(count, (unsigned int)__popcll(mask));
-            nb = bcast(nb, 0);
-            if (ready) list[nb + lanes_below(mask)] = i;
-        }
+    for (int base = blockIdx.x * blockDim.x; base < a.n; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        list_append(a, list_set(a, 0), a.list0, i < a.n - 1 && dep[i] == 0, i);
     }
 }
 
@@ -545,7 +590,7 @@ void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s
 {
     hipLaunchKernelGGL(k_factor_init, dim3(1024), dim3(256), 0, s, a.n, dep_init, a.dep, a.head, a.level,
                        a.out_cnt, a.out_start, a.diag);
-    hipLaunchKernelGGL(k_factor_seed, dim3(1024), dim3(256), 0, s, a.n, (const int *)a.dep, a.list0, a.round_count);
+    hipLaunchKernelGGL(k_factor_seed, dim3(1024), dim3(256), 0, s, a, (const int *)a.dep);
 }
 
 void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s)

@@ -43,14 +43,27 @@ constexpr int kFillLists = 32;
 constexpr int kBigGrid = 256;
 constexpr int kBigCap = 2048;
 
+// Ready lists are split into kListSegs segments of seg_cap entries (workgroup
+// b appends to segment b % kListSegs) and an overflow region of n entries at
+// the end: one shared append counter would serialize every append of a round
+// (returning atomics on one address retire one at a time, ~11 ns apart on
+// MI355X).  A round's counters are kListSlots words (the segments, then the
+// overflow region); three sets rotate (round % 3).
+constexpr int kListSegs = 63;
+constexpr int kListSlots = kListSegs + 1;
+inline long long list_seg_cap(int n) { return 2LL * ((n + kListSegs - 1) / kListSegs) + 1024; }
+inline long long list_capacity(int n) { return (long long)kListSegs * list_seg_cap(n) + n; }
+
 struct FactorArgs {
     int n;
     const int *lo_ptr;      // n+1, strict-lower part of the Laplacian (CSC)
     const int *lo_row;
     const double *lo_w;     // |A[r,k]|
     int *dep, *head, *level;   // head: n * kFillLists
-    int *list0, *list1;     // ready lists of even / odd rounds
-    unsigned int *round_count;   // [round] = columns eliminated in that round
+    int *list0, *list1;     // ready lists of even / odd rounds (list_capacity(n) entries)
+    unsigned int *seg_count;   // 3 x kListSlots append counters (set round % 3 = that round's list)
+    int seg_cap;            // list_seg_cap(n)
+    unsigned int *round_count;   // [round] = columns eliminated in that round (written by the round)
     long long *wave_state;  // per workgroup: fill cursor, fill end, out cursor, out end
     unsigned long long *ctr;
     FillRec *pool;

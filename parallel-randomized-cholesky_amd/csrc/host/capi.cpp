@@ -87,7 +87,7 @@ struct rc_handle_s {
     // factor inputs / working set
     DevVec lo_ptr, lo_row, lo_w, dep_init;
     DevVec big_list, big_count;
-    DevVec dep, head, level, list0, list1, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
+    DevVec dep, head, level, list0, list1, seg_count, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
         out_cnt, diag64, scratch;
     long long pool_cap = 0, out_cap = 0;
     int scratch_cap = 0;
@@ -724,8 +724,9 @@ static int alloc_factor_work(rc_handle_t h)
     RC_HIP(h->dep.ensure(nf * sizeof(int)));
     RC_HIP(h->head.ensure(nf * kFillLists * sizeof(int)));
     RC_HIP(h->level.ensure(nf * sizeof(int)));
-    RC_HIP(h->list0.ensure(nf * sizeof(int)));
-    RC_HIP(h->list1.ensure(nf * sizeof(int)));
+    RC_HIP(h->list0.ensure((size_t)list_capacity(h->nf) * sizeof(int)));
+    RC_HIP(h->list1.ensure((size_t)list_capacity(h->nf) * sizeof(int)));
+    RC_HIP(h->seg_count.ensure(3 * kListSlots * sizeof(unsigned int)));
     RC_HIP(h->round_count.ensure((nf + 2) * sizeof(unsigned int)));
     RC_HIP(h->big_count.ensure((nf + 2) * sizeof(unsigned int)));
     RC_HIP(h->big_list.ensure(nf * sizeof(int)));
@@ -754,6 +755,8 @@ static FactorArgs factor_args(rc_handle_t h)
     a.level = h->level.as<int>();
     a.list0 = h->list0.as<int>();
     a.list1 = h->list1.as<int>();
+    a.seg_count = h->seg_count.as<unsigned int>();
+    a.seg_cap = (int)list_seg_cap(h->nf);
     a.round_count = h->round_count.as<unsigned int>();
     a.wave_state = h->wave_state.as<long long>();
     a.ctr = h->ctr.as<unsigned long long>();
@@ -794,27 +797,30 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         RC_HIP(hipEventRecord(e0, s));
         RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), s));
         RC_HIP(hipMemsetAsync(h->round_count.p, 0, ((size_t)h->nf + 2) * sizeof(unsigned int), s));
+        RC_HIP(hipMemsetAsync(h->seg_count.p, 0, 3 * kListSlots * sizeof(unsigned int), s));
         RC_HIP(hipMemsetAsync(h->big_count.p, 0, ((size_t)h->nf + 2) * sizeof(unsigned int), s));
         RC_HIP(hipMemsetAsync(h->wave_state.p, 0, (size_t)(h->num_blocks + kBigGrid) * 4 * sizeof(long long), s));
         launch_factor_setup(a, h->dep_init.as<int>(), s);
         // rounds in growing batches; a round whose list is empty returns at once
         int round = 0, batch = 8;
         unsigned long long st = 0;
-        unsigned int *probe = nullptr;
-        RC_HIP(hipHostMalloc((void **)&probe, 4 * sizeof(unsigned int), hipHostMallocDefault));
+        unsigned int *probe = nullptr;     // the next round's list counters, then the status word
+        RC_HIP(hipHostMalloc((void **)&probe, (kListSlots + 2) * sizeof(unsigned int), hipHostMallocDefault));
         while (true) {
             int nr = std::min(batch, h->nf + 1 - round);
             if (nr <= 0) break;
             launch_factor_rounds(a, h->num_blocks, round, nr, s);
             round += nr;
-            hipError_t e = hipMemcpyAsync(probe, h->round_count.as<unsigned int>() + round, sizeof(unsigned int),
-                                          hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(probe + 2, h->ctr.as<unsigned long long>() + kCtrStatus,
+            hipError_t e = hipMemcpyAsync(probe, h->seg_count.as<unsigned int>() + (size_t)(round % 3) * kListSlots,
+                                          kListSlots * sizeof(unsigned int), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(probe + kListSlots, h->ctr.as<unsigned long long>() + kCtrStatus,
                                                     sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) { (void)hipHostFree(probe); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
-            std::memcpy(&st, probe + 2, sizeof(st));
-            if (st != 0 || probe[0] == 0) break;
+            std::memcpy(&st, probe + kListSlots, sizeof(st));
+            unsigned long long pending = probe[kListSegs];
+            for (int q = 0; q < kListSegs; ++q) pending += std::min<unsigned long long>(probe[q], (unsigned long long)a.seg_cap);
+            if (st != 0 || pending == 0) break;
             batch = std::min(batch * 2, 256);
         }
         (void)hipHostFree(probe);
