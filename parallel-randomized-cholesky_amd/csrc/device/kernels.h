@@ -243,6 +243,22 @@ struct PcgScalars {
 };
 enum { kPcgOk = 0, kPcgBreakdown = 1, kPcgTimeout = 2 };
 
+// Chunk claims of the persistent sweeps go to kClaimQueues counters
+// kClaimStride words apart: chunk c belongs to queue c % K (K = min(queues,
+// grid)) and workgroup b claims from queue b % K, so claims do not serialize
+// on one address (returning atomics on one address retire ~11 ns apart).
+// Forward counters first, then the backward ones.
+constexpr int kClaimQueues = 16;
+constexpr int kClaimStride = 64;
+constexpr int kClaimWords = 2 * kClaimQueues * kClaimStride;
+__device__ __forceinline__ void reset_claims(unsigned int *fwd, unsigned int *bwd, int t)
+{
+    if (t < kClaimQueues) {
+        fwd[t * kClaimStride] = 0u;
+        bwd[t * kClaimStride] = 0u;
+    }
+}
+
 struct SolveArgs {
     int n;                        // rows; index n is a dummy slot holding 0
     // A (PCG system) in sliced ELL over position tiles

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p)
     __shared__ int flag;
     MultiScalars *sc = a.sc;
     if (sc->all_done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int K = a.K;
     const int q = threadIdx.x % K;
     const int rows_per_block = kMB / K;
@@ -178,10 +178,13 @@ __global__ void __launch_bounds__(1024) k_msweep(MultiArgs a, const double *p_cu
     const int *ptr = kFwd ? a.l_ptr : a.lt_ptr;
     const int *ind = kFwd ? a.l_ind : a.lt_ind;
     const T *diag = (const T *)a.diag;
-    unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
+    // claim queue of this workgroup: its j-th claim is chunk j * nq + cq
+    const int nq = (int)gridDim.x < kClaimQueues ? (int)gridDim.x : kClaimQueues;
+    const int cq = (int)(blockIdx.x % (unsigned int)nq);
+    unsigned int *ctr = (kFwd ? a.ctr_fwd : a.ctr_bwd) + cq * kClaimStride;
     while (true) {
         if (t == 0) {
-            s_chunk = (int)atomicAdd(ctr, 1u);
+            s_chunk = (int)atomicAdd(ctr, 1u) * nq + cq;
             s_stop = (ld_relaxed(&sc->status) & kPcgTimeout) ? 1 : 0;
         }
         __syncthreads();
@@ -395,7 +398,7 @@ __global__ void k_mprepare(MultiArgs a)
         ((T *)a.y)[i] = dummy ? (T)0 : sent_value(T());
         ((T *)a.z)[i] = dummy ? (T)0 : sent_value(T());
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
 }
 
 // b (positions, K-strided) from the user's column-major B (n x k):

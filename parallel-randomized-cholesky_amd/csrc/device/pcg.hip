@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
     __shared__ int flag;
     PcgScalars *sc = a.sc;
     if (sc->done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int lane = lane_id();
     double acc_dot = 0.0;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ntiles * kWave; q += gridDim.x * blockDim.x) {
@@ -321,7 +321,10 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     const double alpha = (kFwd && update_rx) ? sc->alpha : 0.0;
-    unsigned int *ctr = kFwd ? a.ctr_fwd : a.ctr_bwd;
+    // claim queue of this workgroup: its j-th claim is chunk j * nq + q
+    const int nq = (int)gridDim.x < kClaimQueues ? (int)gridDim.x : kClaimQueues;
+    const int q = (int)(blockIdx.x % (unsigned int)nq);
+    unsigned int *ctr = (kFwd ? a.ctr_fwd : a.ctr_bwd) + q * kClaimStride;
     const int4 *meta = kFwd ? a.cm_f : a.cm_b;
     const int nb = a.bulk_chunks;
     const int t = threadIdx.x;
@@ -340,9 +343,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
     int c_pend = 0;
     unsigned int st_pend = 0;
     if (t == 0) {
-        s_claim[0] = (int)atomicAdd(ctr, 1u);
-        s_claim[1] = (int)atomicAdd(ctr, 1u);
-        c_pend = (int)atomicAdd(ctr, 1u);
+        s_claim[0] = (int)atomicAdd(ctr, 1u) * nq + q;
+        s_claim[1] = (int)atomicAdd(ctr, 1u) * nq + q;
+        c_pend = (int)atomicAdd(ctr, 1u) * nq + q;
         st_pend = ld_relaxed(&sc->status);
     }
     __syncthreads();
@@ -361,7 +364,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
             // this atomic and the status load complete during the next chunk
             s_claim[par] = c_pend;              // slots alternate, so one barrier suffices
             s_stop = (int)st_pend;
-            c_pend = (int)atomicAdd(ctr, 1u);
+            c_pend = (int)atomicAdd(ctr, 1u) * nq + q;
             st_pend = ld_relaxed(&sc->status);
         }
         __syncthreads();
@@ -626,9 +629,8 @@ __global__ void k_pcg_prepare(SolveArgs a)
         a.p1[i] = 0.0;
         a.ap[i] = 0.0;
     }
+    if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *a.ctr_fwd = 0u;
-        *a.ctr_bwd = 0u;
         a.x[a.n] = 0.0; a.r[a.n] = 0.0; a.p0[a.n] = 0.0; a.p1[a.n] = 0.0; a.ap[a.n] = 0.0;
         ((T *)a.y)[a.n] = (T)0;     // dummy slot read by ELL padding
         ((T *)a.z)[a.n] = (T)0;
@@ -665,7 +667,7 @@ __global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src, d
 }
 __global__ void k_reset_counters(unsigned int *a, unsigned int *b)
 {
-    if (threadIdx.x == 0) { *a = 0u; *b = 0u; }
+    reset_claims(a, b, threadIdx.x);
 }
 
 int sweep_blocks(const SolveArgs &a)
@@ -713,7 +715,7 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, BatchArgs b, c
 {
     PcgScalars *sc = a.sc;
     if (sc->done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { *a.ctr_fwd = 0u; *a.ctr_bwd = 0u; }
+    if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const T *z = (const T *)a.z;
     const int lane = lane_id();
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {

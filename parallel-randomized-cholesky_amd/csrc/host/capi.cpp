@@ -1455,7 +1455,7 @@ static SolveArgs solve_args(rc_handle_t h)
     a.sc = h->scal.as<PcgScalars>();
     a.partials = h->partials.as<double>();
     a.ctr_fwd = h->ctrs.as<unsigned int>();
-    a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
+    a.ctr_bwd = h->ctrs.as<unsigned int>() + kClaimQueues * kClaimStride;
     a.reduce_blocks = h->reduce_blocks;
     a.sweep_blocks = h->sweep_blocks;
     a.bulk_chunks = h->bulk_chunks;
@@ -1528,7 +1528,7 @@ static int alloc_solve(rc_handle_t h)
     RC_HIP(h->z.ensure(n * s));
     RC_HIP(h->scal.ensure(sizeof(PcgScalars)));
     RC_HIP(h->partials.ensure((size_t)h->reduce_blocks * 2 * sizeof(double)));
-    RC_HIP(h->ctrs.ensure(4 * sizeof(unsigned int)));
+    RC_HIP(h->ctrs.ensure(kClaimWords * sizeof(unsigned int)));
     if ((size_t)h->y.p != old_y) invalidate_graph(h);
     return RC_OK;
 }
@@ -1874,7 +1874,7 @@ static int multi_args(rc_handle_t h, int k, MultiArgs &a)
     RC_HIP(h->m_z.ensure(tot * sz));
     RC_HIP(h->m_sc.ensure(sizeof(MultiScalars)));
     RC_HIP(h->m_part.ensure((size_t)h->reduce_blocks * 2 * K * sizeof(double)));
-    RC_HIP(h->ctrs.ensure(4 * sizeof(unsigned int)));
+    RC_HIP(h->ctrs.ensure(kClaimWords * sizeof(unsigned int)));
     a.n = (int)n;
     a.K = K;
     a.k = k;
@@ -1902,7 +1902,7 @@ static int multi_args(rc_handle_t h, int k, MultiArgs &a)
     a.sc = h->m_sc.as<MultiScalars>();
     a.partials = h->m_part.as<double>();
     a.ctr_fwd = h->ctrs.as<unsigned int>();
-    a.ctr_bwd = h->ctrs.as<unsigned int>() + 1;
+    a.ctr_bwd = h->ctrs.as<unsigned int>() + kClaimQueues * kClaimStride;
     a.reduce_blocks = h->reduce_blocks;
     a.sweep_blocks = h->sweep_blocks;
     a.spin_ticks = h->spin_ticks;
