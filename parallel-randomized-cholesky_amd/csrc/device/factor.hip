@@ -266,9 +266,12 @@ __device__ __forceinline__ void list_append(const FactorArgs &a, unsigned int *s
 
 // Gathers column k's strict-lower entries and its pending fills (kFillLists
 // linked lists, walked in lockstep, one lane per list; f = this lane's list
-// head) into b, writing the first `cap` entries; returns the total count.
+// head) into b, writing the first `cap` entries; returns the total count, or
+// with kStopOverCap any count above `cap` as soon as one is reached (the
+// column is then handed on without walking the rest of its lists).
 // The order of the gathered entries does not matter: the (row, weight) sort
 // makes it canonical.
+template <bool kStopOverCap>
 __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int norig, int f, const ColBuf &b, int cap)
 {
     const int lane = lane_id();
@@ -279,6 +282,7 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
     }
     int t = norig;
     while (true) {
+        if (kStopOverCap && t > cap) break;
         const bool act = f >= 0;
         const uint64_t mask = ballot(act);
         if (!mask) break;
@@ -461,7 +465,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     // counter is kept), and a column that turns out larger than the buffer is
     // deferred (round kernel) or gathered again into the global scratch
     const ColBuf b = carve(lds, lds_cap);
-    int m = gather_column(a, lo0, norig, f0, b, lds_cap);
+    int m = gather_column<kDefer>(a, lo0, norig, f0, b, lds_cap);
     if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
         if (lane == 0) {
             a.level[k] = round;
@@ -486,7 +490,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
                 return false;
             }
             const ColBuf g = carve(a.scratch + (size_t)blockIdx.x * (size_t)a.scratch_cap * kBytesPerEntry, a.scratch_cap);
-            if (gather_column(a, lo0, norig, f0, g, a.scratch_cap) != m) {
+            if (gather_column<false>(a, lo0, norig, f0, g, a.scratch_cap) != m) {
                 if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
                 return false;
             }

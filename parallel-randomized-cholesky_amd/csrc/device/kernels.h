@@ -35,7 +35,7 @@ enum : unsigned long long {
 // Pending fills of a column are kept in kFillLists singly linked lists
 // (head[k * kFillLists + h]; a record goes to list (record index mod
 // kFillLists)), so a column's gather walks them in parallel.
-constexpr int kFillLists = 32;
+constexpr int kFillLists = 64;
 
 // Columns with more than the round kernel's LDS capacity (128 entries) are
 // deferred to a second kernel of the same round: kBigGrid one-wave workgroups
