@@ -161,12 +161,15 @@ def cpu_baseline(wl, cfg, b_full, tol, max_n):
     t1 = time.perf_counter()
     x, st = orc.pcg(sys_, f, b[perm], tol=tol, precision=cfg["prec"])
     t2 = time.perf_counter()
+    nrhs = cfg.get("nrhs", 1)
     sample = (f"oracle (1 thread) factor+PCG on {'x'.join(map(str, sdims))} "
-              f"({n} rows): factor {t1 - t0:.2f}s + PCG {t2 - t1:.2f}s, {st['iterations']} iterations")
+              f"({n} rows, nnz-sort order): factor {t1 - t0:.2f}s + PCG {t2 - t1:.2f}s, {st['iterations']} iterations")
+    if nrhs > 1:
+        sample += f"; one right-hand side solved, PCG time x{nrhs} for the {nrhs} right-hand sides"
     if scale != 1.0:
         sample += f"; scaled x{scale:.1f} to {full_n} rows"
-    return {"value": (t2 - t0) * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
-            "iterations": st["iterations"], "factor_s": t1 - t0, "pcg_s": t2 - t1}
+    return {"value": ((t1 - t0) + nrhs * (t2 - t1)) * scale, "unit": "s", "cores": 1, "kind": "port",
+            "sample": sample, "iterations": st["iterations"], "factor_s": t1 - t0, "pcg_s": t2 - t1}
 
 
 def main():
