@@ -124,11 +124,33 @@ __device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int
 }
 
 // Sum over the G lanes of a row group (G a power of two <= 64, groups aligned
-// inside the wave): fixed butterfly, so the result is deterministic.
+// inside the wave), valid in the group's LAST lane: a shift-and-add scan over
+// DPP row shifts (1, 2, 4, 8 lanes inside 16-lane rows) and row broadcasts
+// (lane 15 to the next row, lane 31 to the upper half).  The lanes move data
+// through the VALU, not through LDS like a shuffle; the order is fixed, so the
+// result is deterministic.  Every lane of the wave must be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, kRowMask, 0xF, true));
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), kCtrl, kRowMask, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, kRowMask, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <typename T>
 __device__ __forceinline__ T group_sum(T v, int G)
 {
-    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (G >= 2) v += dpp_move<0x111, 0xF>(v);     // row_shr:1
+    if (G >= 4) v += dpp_move<0x112, 0xF>(v);     // row_shr:2
+    if (G >= 8) v += dpp_move<0x114, 0xF>(v);     // row_shr:4
+    if (G >= 16) v += dpp_move<0x118, 0xF>(v);    // row_shr:8
+    if (G >= 32) v += dpp_move<0x142, 0xA>(v);    // row_bcast:15 into rows 1 and 3
+    if (G >= 64) v += dpp_move<0x143, 0xC>(v);    // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -241,7 +263,7 @@ __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, dou
     const int stride = sweep_rows_pow2(cnt) * G;
     const int i = t / G, g = t - i * G;
     const bool active = t < stride && i < cnt;
-    const bool lead = active && g == 0;
+    const bool lead = active && g == G - 1;     // the lane that receives the group sum
     const int k = m.x + i;
     T acc = (T)0;
     double rk = 0.0, apk = 0.0, xk = 0.0, pk = 0.0;
@@ -285,7 +307,7 @@ __device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
     const int stride = sweep_rows_pow2(cnt) * G;
     const int i = t / G, g = t - i * G;
     const bool active = t < stride && i < cnt;
-    const bool lead = active && g == 0;
+    const bool lead = active && g == G - 1;     // the lane that receives the group sum
     const int k = m.x + i;
     T acc = (T)0, yk = (T)0, dk = (T)0;
     if (lead) {
@@ -400,7 +422,7 @@ __global__ void __launch_bounds__(kRedBlock) k_tail_prep(SolveArgs a, const doub
         T acc = (T)0;
         for (int e = a.tp_ptr[q] + lane; e < a.tp_ptr[q + 1]; e += kWave) acc = fma_t(-val[e], y[a.tp_ind[e]], acc);
         acc = group_sum(acc, kWave);
-        if (lane == 0) {
+        if (lane == kWave - 1) {
             const int k = a.tail_pos0 + q;
             double rk = a.r[k];
             if (update_rx) {
@@ -484,7 +506,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         const int G = tail_group(d.y);
         const int stride = sweep_rows_pow2(d.y) * G;
         const int i = t / G, g = t - i * G;
-        const bool lead = t < stride && i < d.y && g == 0;
+        const bool lead = t < stride && i < d.y && g == G - 1;
         const int q = d.x - tp0 + i;
         T acc = (T)0;
 #pragma unroll
