@@ -263,17 +263,29 @@ __global__ void k_ell_fill(int nslices, const int *start, const int *cnt, const 
         int len = row >= 0 ? src.ptr[row + 1] - b : 0;
         const int w = width[c];
         const int o = off[c];
-        for (int t = 0; t < w; ++t) {
-            int ind = pad_ind;
-            TD v = pad_val;
-            if (t < len) {
-                int e = src.eidx ? src.eidx[b + t] : b + t;
-                int col = src.rowk ? src.rowk[e] : src.ind[e];
-                ind = src.colmap ? src.colmap[col] : col;
-                v = (TD)sval[e];
+        // four entries per lane in flight (index, column and value, position)
+        constexpr int U = 4;
+        for (int t0 = 0; t0 < w; t0 += U) {
+            int e[U], col[U];
+            TD v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u;
+                e[u] = t < len ? (src.eidx ? src.eidx[b + t] : b + t) : -1;
             }
-            ell_ind[o + t * kWave + lane] = ind;
-            ell_val[o + t * kWave + lane] = v;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                col[u] = e[u] >= 0 ? (src.rowk ? src.rowk[e[u]] : src.ind[e[u]]) : 0;
+                v[u] = e[u] >= 0 ? (TD)sval[e[u]] : pad_val;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u;
+                if (t < w) {
+                    ell_ind[o + t * kWave + lane] = e[u] >= 0 ? (src.colmap ? src.colmap[col[u]] : col[u]) : pad_ind;
+                    ell_val[o + t * kWave + lane] = v[u];
+                }
+            }
         }
     }
 }
@@ -481,30 +493,48 @@ __global__ void k_sweep_fill(int nchunks, const int *start, const int *cnt, cons
     const int nwaves = gridDim.x * blockDim.x / kWave;
     const TS *sval = (const TS *)src.val;
     for (int c = wave; c < nchunks; c += nwaves) {
-        const int nc = cnt[c];
+        const int nc = cnt[c];                  // <= 64 rows (sweep chunks)
         const int P = sweep_rows_pow2(nc);
         const int G = sweep_group(nc);
         const int stride = P * G;
         const int total = slices[c] * stride;
         const int o = off[c];
-        for (int idx = lane; idx < total; idx += kWave) {
-            int sl = idx / stride, t = idx - sl * stride;
-            int i = t / G, g = t - i * G;
-            int et = sl * G + g;
-            int ind = pad_ind;
-            TD v = pad_val;
-            if (i < nc) {
-                int row = order[start[c] + i];
-                int b = src.ptr[row], len = src.ptr[row + 1] - b;
-                if (et < len) {
-                    int e = src.eidx ? src.eidx[b + et] : b + et;
-                    int col = src.rowk ? src.rowk[e] : src.ind[e];
-                    ind = src.colmap ? src.colmap[col] : col;
-                    v = (TD)sval[e];
+        // lane i holds row i's source range
+        int rb = 0, rl = 0;
+        if (lane < nc) {
+            const int row = order[start[c] + lane];
+            rb = src.ptr[row];
+            rl = src.ptr[row + 1] - rb;
+        }
+        // four output slots per lane in flight: index, then column and value,
+        // then the column's position
+        constexpr int U = 4;
+        for (int idx0 = 0; idx0 < total; idx0 += U * kWave) {
+            int e[U], col[U];
+            TD v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = idx0 + u * kWave + lane;
+                const int sl = idx / stride, t = idx - sl * stride;
+                const int i = t / G, g = t - i * G;
+                const int et = sl * G + g;
+                const int b = __shfl(rb, i & (kWave - 1), kWave);
+                const int len = __shfl(rl, i & (kWave - 1), kWave);
+                e[u] = (idx < total && i < nc && et < len) ? (src.eidx ? src.eidx[b + et] : b + et) : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                col[u] = e[u] >= 0 ? (src.rowk ? src.rowk[e[u]] : src.ind[e[u]]) : 0;
+                v[u] = e[u] >= 0 ? (TD)sval[e[u]] : pad_val;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = idx0 + u * kWave + lane;
+                if (idx < total) {
+                    ell_ind[o + idx] = e[u] >= 0 ? (src.colmap ? src.colmap[col[u]] : col[u]) : pad_ind;
+                    ell_val[o + idx] = v[u];
                 }
             }
-            ell_ind[o + idx] = ind;
-            ell_val[o + idx] = v;
         }
     }
 }
