@@ -113,16 +113,20 @@ __global__ void k_lt_counts(AssembleArgs a, int *cnt)
 }
 
 template <typename T>
+// One 16-lane group per row: the row's entries are read and written as
+// contiguous runs.
 __global__ void k_lt_fill(AssembleArgs a, const int *lt_ptr, int *lt_ind, T *lt_val, T *diag_out)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < a.ns; k += gridDim.x * blockDim.x) {
-        long long src = a.out_start[k];
-        int dst = lt_ptr[k], len = lt_ptr[k + 1] - dst;
-        for (int t = 0; t < len; ++t) {
+    const int g = threadIdx.x & 15;
+    const int ngroups = (gridDim.x * blockDim.x) >> 4;
+    for (int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; k < a.ns; k += ngroups) {
+        const long long src = a.out_start[k];
+        const int dst = lt_ptr[k], len = lt_ptr[k + 1] - dst;
+        for (int t = g; t < len; t += 16) {
             lt_ind[dst + t] = a.out_row[src + t];
             lt_val[dst + t] = (T)a.out_val[src + t];
         }
-        diag_out[k] = (T)a.diag_in[k];
+        if (g == 0) diag_out[k] = (T)a.diag_in[k];
     }
 }
 
@@ -150,8 +154,10 @@ __global__ void k_copy_keys(const int *src, uint32_t *dst, int n)
 
 __global__ void k_row_expand(int ns, const int *ptr, int *rowk)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += gridDim.x * blockDim.x)
-        for (int e = ptr[k]; e < ptr[k + 1]; ++e) rowk[e] = k;
+    const int g = threadIdx.x & 15;         // one 16-lane group per row
+    const int ngroups = (gridDim.x * blockDim.x) >> 4;
+    for (int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; k < ns; k += ngroups)
+        for (int e = ptr[k] + g; e < ptr[k + 1]; e += 16) rowk[e] = k;
 }
 
 __global__ void k_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr)
@@ -290,31 +296,50 @@ __global__ void k_ell_fill(int nslices, const int *start, const int *cnt, const 
     }
 }
 
+// Widest L / L^T row of every level.  Positions are sorted by level, so a
+// wave walks a contiguous span of kLevelSpan positions keeping the running
+// maximum of the current level and adds it with one atomicMax when the level
+// changes (a wide level's words would otherwise take one returning atomic per
+// 64 rows, all on one address).
+constexpr int kLevelSpan = 1024;
 __global__ void k_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw)
 {
-    const int first = (blockIdx.x * blockDim.x + threadIdx.x) & ~(kWave - 1);
-    for (int base = first; base < ns; base += gridDim.x * blockDim.x) {
-        // the loop bound is uniform over the wave; lanes past ns count as
-        // width 0 at level -1
-        const int p = base + lane_id();
-        int L = -1, w = 0;
-        if (p < ns) {
-            const int row = order[p];
-            L = level[row];
-            const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
-            w = a > b ? a : b;
-        }
-        const int L0 = __shfl(L, 0, kWave);
-        const bool uniform = __all(L == L0 || L < 0);
-        if (uniform) {
-            for (int off = 32; off > 0; off >>= 1) {
-                const int o = __shfl_xor(w, off, kWave);
-                w = o > w ? o : w;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int nwaves = gridDim.x * blockDim.x / kWave;
+    for (int s0 = wave * kLevelSpan; s0 < ns; s0 += nwaves * kLevelSpan) {
+        int curL = -1, curW = 0;            // wave-uniform
+        for (int base = s0; base < s0 + kLevelSpan && base < ns; base += kWave) {
+            // lanes past ns count as width 0 at level -1
+            const int p = base + lane_id();
+            int L = -1, w = 0;
+            if (p < ns) {
+                const int row = order[p];
+                L = level[row];
+                const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
+                w = a > b ? a : b;
             }
-            if (lane_id() == 0 && L0 >= 0) atomicMax(lw + L0, w);
-        } else if (L >= 0) {
-            atomicMax(lw + L, w);
+            const int L0 = __shfl(L, 0, kWave);
+            const bool uniform = __all(L == L0 || L < 0);
+            if (uniform) {
+                for (int off = 32; off > 0; off >>= 1) {
+                    const int o = __shfl_xor(w, off, kWave);
+                    w = o > w ? o : w;
+                }
+                if (L0 != curL) {
+                    if (curL >= 0 && lane_id() == 0) atomicMax(lw + curL, curW);
+                    curL = L0;
+                    curW = w;
+                } else {
+                    curW = w > curW ? w : curW;
+                }
+            } else {
+                if (curL >= 0 && lane_id() == 0) atomicMax(lw + curL, curW);
+                curL = -1;
+                curW = 0;
+                if (L >= 0) atomicMax(lw + L, w);
+            }
         }
+        if (curL >= 0 && lane_id() == 0) atomicMax(lw + curL, curW);
     }
 }
 
