@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
         list_set(a, round + 2)[lane_id()] = 0u;
         if (lane_id() == 0) a.round_count[round] = ncur;
     }
-    if (ncur == 0) return;
+    if (ncur <= (unsigned int)kBigGrid) return;     // a narrow round: the large-LDS kernel takes all of it
     if (status != 0ull) return;
     Cursor fcur, ocur;
     long long *ws = a.wave_state + 4 * (size_t)blockIdx.x;
@@ -540,13 +540,18 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
 // The round's columns deferred by k_factor_round (more than kLdsCap entries):
 // one wave per column with kBigCap entries of LDS (columns above that use the
 // per-workgroup global scratch).  Runs after k_factor_round of the same round.
-__global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int round, int *next)
+__global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int round, const int *cur, int *next)
 {
     extern __shared__ __attribute__((aligned(16))) char lds_big[];
-    const unsigned int nbig = a.big_count[round];
+    // a round of at most kBigGrid columns is eliminated here entirely (one
+    // column per workgroup); otherwise the columns k_factor_round deferred
+    unsigned int c, incl;
+    list_counts(a, list_set(a, round), c, incl);
+    const unsigned int ndef = a.big_count[round];
     const unsigned long long status = a.ctr[kCtrStatus];
-    const int first = a.big_list[min(blockIdx.x, (unsigned int)a.n - 1u)];   // used only below the count
-    asm volatile("" ::"s"(first), "s"(nbig));
+    const unsigned int ncur = __shfl(incl, kWave - 1, kWave);
+    const bool narrow = ncur <= (unsigned int)kBigGrid;
+    const unsigned int nbig = narrow ? ncur : ndef;
     if (nbig == 0) return;
     if (status != 0ull) return;
     Cursor fcur, ocur;
@@ -554,7 +559,7 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     fcur.cur = ws[0]; fcur.end = ws[1];
     ocur.cur = ws[2]; ocur.end = ws[3];
     for (unsigned int i = blockIdx.x; i < nbig; i += gridDim.x) {
-        const int col = i == blockIdx.x ? first : a.big_list[i];
+        const int col = narrow ? cur[list_index(a, c, incl, i)] : a.big_list[i];
         if (!eliminate_column<2, false>(a, col, round, lds_big, kBigCap, fcur, ocur, next, list_set(a, round + 1), nullptr))
             break;
     }
@@ -604,7 +609,7 @@ void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, 
         int *next = (r & 1) ? a.list0 : a.list1;
         hipLaunchKernelGGL(k_factor_round, dim3(num_blocks), dim3(kWave), 0, s, a, r, cur, next);
         hipLaunchKernelGGL(k_factor_round_big, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, a, r,
-                           next);
+                           cur, next);
     }
 }
 

@@ -39,7 +39,8 @@ constexpr int kFillLists = 64;
 
 // Columns with more than the round kernel's LDS capacity (128 entries) are
 // deferred to a second kernel of the same round: kBigGrid one-wave workgroups
-// with kBigCap entries of LDS each.
+// with kBigCap entries of LDS each.  A round of at most kBigGrid columns is
+// eliminated by that second kernel entirely.
 constexpr int kBigGrid = 256;
 constexpr int kBigCap = 2048;
 
