@@ -152,12 +152,15 @@ __global__ void k_copy_keys(const int *src, uint32_t *dst, int n)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = (uint32_t)src[i];
 }
 
-__global__ void k_row_expand(int ns, const int *ptr, int *rowk)
+// rowk[e] = map[k] (or k) for every entry e of row k.
+__global__ void k_row_expand(int ns, const int *ptr, const int *map, int *rowk)
 {
     const int g = threadIdx.x & 15;         // one 16-lane group per row
     const int ngroups = (gridDim.x * blockDim.x) >> 4;
-    for (int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; k < ns; k += ngroups)
-        for (int e = ptr[k] + g; e < ptr[k + 1]; e += 16) rowk[e] = k;
+    for (int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; k < ns; k += ngroups) {
+        const int v = map ? map[k] : k;
+        for (int e = ptr[k] + g; e < ptr[k + 1]; e += 16) rowk[e] = v;
+    }
 }
 
 __global__ void k_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr)
@@ -379,7 +382,7 @@ __device__ __forceinline__ int src_entry(const EllSrc &src, int b, int et, int &
 {
     const int e = src.eidx ? src.eidx[b + et] : b + et;
     const int col = src.rowk ? src.rowk[e] : src.ind[e];
-    pos = src.colmap[col];
+    pos = src.colmap ? src.colmap[col] : col;
     return e;
 }
 
@@ -608,9 +611,9 @@ void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_keys, dim3(2048), dim3(256), 0, s, src, dst, n);
 }
-void launch_row_expand(int ns, const int *ptr, int *rowk, hipStream_t s)
+void launch_row_expand(int ns, const int *ptr, const int *map, int *rowk, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_row_expand, dim3(2048), dim3(256), 0, s, ns, ptr, rowk);
+    hipLaunchKernelGGL(k_row_expand, dim3(2048), dim3(256), 0, s, ns, ptr, map, rowk);
 }
 void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s)
 {

@@ -123,7 +123,7 @@ void launch_lt_fill(const AssembleArgs &a, const int *lt_ptr, int *lt_ind, T *lt
 // small helpers
 void launch_iota(int *v, int n, hipStream_t s);
 void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s);
-void launch_row_expand(int ns, const int *ptr, int *rowk, hipStream_t s);
+void launch_row_expand(int ns, const int *ptr, const int *map, int *rowk, hipStream_t s);
 // ptr[r] = first index i with (keys[i] >> shift) >= r, r in [0, nseg]; keys sorted.
 void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s);
 void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s);

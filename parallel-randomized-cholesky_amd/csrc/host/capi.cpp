@@ -931,7 +931,6 @@ static int assemble(rc_handle_t h, double *ms_out)
     launch_iota(h->vals0.as<int>(), nnz, s);
     radix_sort_pairs(k0, h->vals0.as<int>(), k1, h->vals1.as<int>(), nnz, bit_width(ns), h->tmp_b.as<int>(), s);
     launch_segment_bounds(nnz, k0, 0, ns, h->l_ptr.as<int>(), s);
-    launch_row_expand(ns, h->lt_ptr.as<int>(), h->rowk.as<int>(), s);
 
     // 3. level order: rows sorted by (level, descending length of their L row)
     unsigned long long *d_max = h->ctr.as<unsigned long long>() + kCtrCount - 1;
@@ -980,6 +979,9 @@ static int assemble(rc_handle_t h, double *ms_out)
                          s);
     }
     launch_inverse_perm(ns, h->order.as<int>(), h->pos_of.as<int>(), s);
+    // position of the L^T row of every entry (the column of its L row): the
+    // forward layouts read it directly instead of mapping per entry
+    launch_row_expand(ns, h->lt_ptr.as<int>(), h->pos_of.as<int>(), h->rowk.as<int>(), s);
     const size_t nlsz = (size_t)nl + 1;
     int *lptr = h->tmp_a.as<int>();
     int *lcnt = lptr + nlsz;
@@ -1055,7 +1057,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->lf_val.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(T)));
     RC_HIP(h->lb_ind.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(int)));
     RC_HIP(h->lb_val.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(T)));
-    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), h->pos_of.as<int>()};
+    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), nullptr};
     launch_sweep_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), fs,
                             h->lf_off.as<int>(), h->lf_width.as<int>(), ns, (T)0, h->lf_ind.as<int>(), h->lf_val.as<T>(),
                             s);
@@ -1839,7 +1841,7 @@ static int build_multi(rc_handle_t h)
     if (rc != RC_OK) return rc;
     launch_pos_csr_fill<double, double>(ns, h->order.as<int>(), as, h->m_aptr.as<int>(), h->m_aind.as<int>(),
                                         h->m_aval.as<double>(), s);
-    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), h->pos_of.as<int>()};
+    EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), nullptr};
     rc = pos_csr(h->l_ptr.as<int>(), fs, h->m_lptr, h->m_lind, h->m_lval, sizeof(T));
     if (rc != RC_OK) return rc;
     launch_pos_csr_fill<T, T>(ns, h->order.as<int>(), fs, h->m_lptr.as<int>(), h->m_lind.as<int>(),
