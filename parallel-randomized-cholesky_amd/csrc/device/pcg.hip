@@ -512,8 +512,22 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
 #pragma unroll
         for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
         if (t < stride) {
-            for (int sl = kTailPref; sl < d.w; ++sl)
-                acc = fma_t(-val[d.z + t + sl * stride], v[ind[d.z + t + sl * stride]], acc);
+            // slices past the prefetched ones (long rows), eight loads in
+            // flight at a time; a masked slot adds -0 * v = -0, which leaves
+            // acc unchanged
+            for (int sl = kTailPref; sl < d.w; sl += 8) {
+                int jj[8];
+                T ll[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool has = sl + u < d.w;
+                    const int e = has ? d.z + t + (sl + u) * stride : 0;
+                    jj[u] = ind[e];
+                    ll[u] = val[e] * (has ? (T)1 : (T)0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
+            }
         }
         if (lead) acc += v[q];
         acc = group_sum(acc, G);
