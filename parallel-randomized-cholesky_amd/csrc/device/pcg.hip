@@ -745,22 +745,31 @@ void launch_bwd(const SolveArgs &a, hipStream_t s)
 // its alpha and beta are zero, so x and r keep their values and p stays z.
 enum { kOpPap = 0, kOpRz = 1, kOpBnorm = 2, kOpRes = 3 };
 
+// p_new = z + beta_sys p_old per row (streaming; the system's beta is looked
+// up once per row instead of once per matrix entry in the SpMV gather).
 template <typename T>
-__global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, BatchArgs b, const double *p_old, double *p_new,
-                                                     int first)
+__global__ void __launch_bounds__(kRedBlock) k_bpupdate(SolveArgs a, BatchArgs b, const double *p_old, double *p_new)
+{
+    PcgScalars *sc = a.sc;
+    if (sc->done) return;
+    const T *z = (const T *)a.z;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+        p_new[i] = b.beta[b.sys_of[i]] * p_old[i] + (double)z[i];
+}
+
+// Ap = A p over the block-diagonal batch (every neighbour of a row belongs
+// to the row's system).
+__global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, const double *p)
 {
     PcgScalars *sc = a.sc;
     if (sc->done) return;
     if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
-    const T *z = (const T *)a.z;
     const int lane = lane_id();
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
-        const int tile = p / kWave;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ntiles * kWave; q += gridDim.x * blockDim.x) {
+        const int tile = q / kWave;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
-        // every neighbour of row p belongs to p's system, so p's beta applies
-        const double beta = (first || p >= a.n) ? 0.0 : b.beta[b.sys_of[p]];
         double acc = 0.0;
         int t = 0;
         for (; t + 4 <= w; t += 4) {
@@ -769,18 +778,12 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, BatchArgs b, c
 #pragma unroll
             for (int u = 0; u < 4; ++u) { j[u] = ind[(t + u) * kWave]; v[u] = val[(t + u) * kWave]; }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = beta * p_old[j[u]] + (double)z[j[u]];
+            for (int u = 0; u < 4; ++u) pj[u] = p[j[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
         }
-        for (; t < w; ++t) {
-            int j = ind[t * kWave];
-            acc = fma_t(val[t * kWave], beta * p_old[j] + (double)z[j], acc);
-        }
-        if (p < a.n) {
-            p_new[p] = beta * p_old[p] + (double)z[p];
-            a.ap[p] = acc;
-        }
+        for (; t < w; ++t) acc = fma_t(val[t * kWave], p[ind[t * kWave]], acc);
+        if (q < a.n) a.ap[q] = acc;
     }
 }
 
@@ -1050,7 +1053,8 @@ void launch_batch_iteration(const SolveArgs &a, const BatchArgs &b, int parity, 
 {
     double *p_old = parity ? a.p1 : a.p0;
     double *p_new = parity ? a.p0 : a.p1;
-    hipLaunchKernelGGL(k_bspmv<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, b, (const double *)p_old, p_new, 0);
+    hipLaunchKernelGGL(k_bpupdate<T>, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, b, (const double *)p_old, p_new);
+    hipLaunchKernelGGL(k_bspmv, dim3(a.reduce_blocks), dim3(kRedBlock), 0, s, a, (const double *)p_new);
     hipLaunchKernelGGL((k_bseg<T, kOpPap>), dim3(batch_grid(b)), dim3(kRedBlock), 0, s, a, b, (const double *)p_new,
                        0);
     launch_fwd<T>(a, (const double *)p_new, 1, s);
