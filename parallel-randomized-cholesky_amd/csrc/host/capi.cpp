@@ -821,7 +821,7 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
             unsigned long long pending = probe[kListSegs];
             for (int q = 0; q < kListSegs; ++q) pending += std::min<unsigned long long>(probe[q], (unsigned long long)a.seg_cap);
             if (st != 0 || pending == 0) break;
-            batch = std::min(batch * 2, 256);
+            batch = std::min(batch * 2, 64);     // bounds the empty rounds launched past the end
         }
         (void)hipHostFree(probe);
         launch_factor_finish(a, s);
