@@ -109,6 +109,7 @@ struct rc_handle_s {
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
     int nap_max = 1;                // measured best: poll again after one s_sleep 2
+    int seg_cap_limit = 0;          // > 0: ready-list segments hold at most this many entries (exercises the overflow region)
     int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
     DevVec tl_start, tl_cnt, tp_ptr, tp_ind, tp_val, tf_off, tf_slices, tf_ind, tf_val, tb_off, tb_slices, tb_ind,
         tb_val, ti_ptr, ti_ind, ti_val, tacc;
@@ -260,6 +261,8 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
         if (nm) h->nap_max = std::max(1, std::min(64, std::atoi(nm)));
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
         if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
+        const char *sc = std::getenv("RCHOL_LIST_SEG_CAP");     // ready-list segment capacity limit (>= 1)
+        if (sc) h->seg_cap_limit = std::max(1, std::atoi(sc));
     }
     *out = h;
     return RC_OK;
@@ -757,6 +760,7 @@ static FactorArgs factor_args(rc_handle_t h)
     a.list1 = h->list1.as<int>();
     a.seg_count = h->seg_count.as<unsigned int>();
     a.seg_cap = (int)list_seg_cap(h->nf);
+    if (h->seg_cap_limit > 0 && h->seg_cap_limit < a.seg_cap) a.seg_cap = h->seg_cap_limit;   // smaller segments fit the lists
     a.round_count = h->round_count.as<unsigned int>();
     a.wave_state = h->wave_state.as<long long>();
     a.ctr = h->ctr.as<unsigned long long>();

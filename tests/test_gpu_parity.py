@@ -149,6 +149,21 @@ def test_schedule_independence(rc, wl):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("cap", [1, 7])
+def test_ready_list_overflow_gives_same_factor(rc, wl, cap, monkeypatch):
+    """Ready-list segments shrunk to `cap` entries: nearly every append of a
+    round spills into the overflow region; the factor is unchanged."""
+    say(f"[gpu] ready-list overflow cap={cap}")
+    s1, _, _ = _solver(rc, wl, "grid3d_20_log", 64)
+    s1.factor()
+    monkeypatch.setenv("RCHOL_LIST_SEG_CAP", str(cap))
+    s2, _, _ = _solver(rc, wl, "grid3d_20_log", 64)
+    s2.factor()
+    monkeypatch.delenv("RCHOL_LIST_SEG_CAP")
+    for a, b in zip(s1.get_factor(), s2.get_factor()):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_pool_regrowth_gives_same_factor(rc, wl):
     say("[gpu] pool regrowth")
     s1, _, _ = _solver(rc, wl, "grid3d_64_log", 64)
