@@ -253,7 +253,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     h->sweep_blocks = sweep_occupancy_blocks(o.device);
     h->sweep_blocks_max = h->sweep_blocks;
     {
-        const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" | "length" (default)
+        const char *lo = std::getenv("RCHOL_LEVEL_ORDER");      // "spatial" (default) | "length" | "bucket"
         if (lo && std::string(lo) == "length") h->level_order = 0;
         if (lo && std::string(lo) == "spatial") h->level_order = 1;
         if (lo && std::string(lo) == "bucket") h->level_order = 2;
