@@ -611,6 +611,17 @@ void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_keys, dim3(2048), dim3(256), 0, s, src, dst, n);
 }
+__global__ void k_tile_keys(int ntiles, const int *idx_user, uint32_t *keys, int *vals)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        keys[t] = (uint32_t)idx_user[(size_t)t * kWave];
+        vals[t] = t;
+    }
+}
+void launch_tile_keys(int ntiles, const int *idx_user, uint32_t *keys, int *vals, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tile_keys, dim3(256), dim3(256), 0, s, ntiles, idx_user, keys, vals);
+}
 void launch_row_expand(int ns, const int *ptr, const int *map, int *rowk, hipStream_t s)
 {
     hipLaunchKernelGGL(k_row_expand, dim3(2048), dim3(256), 0, s, ns, ptr, map, rowk);

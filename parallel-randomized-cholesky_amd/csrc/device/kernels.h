@@ -124,6 +124,8 @@ void launch_lt_fill(const AssembleArgs &a, const int *lt_ptr, int *lt_ind, T *lt
 void launch_iota(int *v, int n, hipStream_t s);
 void launch_copy_keys(const int *src, uint32_t *dst, int n, hipStream_t s);
 void launch_row_expand(int ns, const int *ptr, const int *map, int *rowk, hipStream_t s);
+// keys[t] = idx_user[64 t] (user index of tile t's first position), vals[t] = t
+void launch_tile_keys(int ntiles, const int *idx_user, uint32_t *keys, int *vals, hipStream_t s);
 // ptr[r] = first index i with (keys[i] >> shift) >= r, r in [0, nseg]; keys sorted.
 void launch_segment_bounds(int n, const uint32_t *keys, int shift, int nseg, int *ptr, hipStream_t s);
 void launch_level_keys(int ns, const int *level, const int *lptr, int len_bits, uint32_t *keys, hipStream_t s);
@@ -264,6 +266,7 @@ struct SolveArgs {
     int n;                        // rows; index n is a dummy slot holding 0
     // A (PCG system) in sliced ELL over position tiles
     int ntiles;
+    const int *a_tile_ord;        // order in which the SpMVs visit the tiles (a permutation of 0..ntiles)
     const int *a_off, *a_width, *a_ind;
     const double *a_val;
     // level schedule (positions)

@@ -179,8 +179,9 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
     if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int lane = lane_id();
     double acc_dot = 0.0;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ntiles * kWave; q += gridDim.x * blockDim.x) {
-        const int tile = q / kWave;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+        const int tile = a.a_tile_ord[it / kWave];
+        const int q = tile * kWave + lane;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
@@ -224,8 +225,9 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
 __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const double *x, double *y)
 {
     const int lane = lane_id();
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
-        const int tile = p / kWave;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+        const int tile = a.a_tile_ord[it / kWave];
+        const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
@@ -618,8 +620,9 @@ __global__ void __launch_bounds__(kRedBlock) k_residual(SolveArgs a)
     PcgScalars *sc = a.sc;
     const int lane = lane_id();
     double acc = 0.0, accb = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
-        const int tile = p / kWave;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+        const int tile = a.a_tile_ord[it / kWave];
+        const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
@@ -765,8 +768,9 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, const double *
     if (sc->done) return;
     if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int lane = lane_id();
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ntiles * kWave; q += gridDim.x * blockDim.x) {
-        const int tile = q / kWave;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+        const int tile = a.a_tile_ord[it / kWave];
+        const int q = tile * kWave + lane;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;
@@ -791,8 +795,9 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, const double *
 __global__ void __launch_bounds__(kRedBlock) k_bresid(SolveArgs a)
 {
     const int lane = lane_id();
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < a.ntiles * kWave; p += gridDim.x * blockDim.x) {
-        const int tile = p / kWave;
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+        const int tile = a.a_tile_ord[it / kWave];
+        const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
         const int *ind = a.a_ind + a.a_off[tile] + lane;
         const double *val = a.a_val + a.a_off[tile] + lane;

@@ -99,6 +99,7 @@ struct rc_handle_s {
     DevVec lt_ptr, lt_ind, lt_val, diagT;
     // solve layout: level-order positions, sliced ELL of L, L^T and A
     DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP, lvl_w, cm_f, cm_b;
+    DevVec tile_ord, tile_k, tile_v, tile_tmp;   // SpMV tile order (by the user index of a tile's first row)
     DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
     DevVec a_off, a_width, a_ell_ind, a_ell_val;
     DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
@@ -1166,6 +1167,17 @@ static int assemble(rc_handle_t h, double *ms_out)
     // user index of every position (RHS gather / solution scatter)
     RC_HIP(h->idx_user.ensure(nsz * sizeof(int)));
     launch_gather_i(h->perm_d.as<int>(), h->order.as<int>(), h->idx_user.as<int>(), ns, s);
+    // the SpMV visits the tiles in the user order of their first rows: a row's
+    // neighbours sit in other levels, i.e. far apart in positions, but near it
+    // in the input numbering, so tiles visited together gather from the same
+    // lines (grids: a few neighbouring planes; batches: one system at a time)
+    RC_HIP(h->tile_ord.ensure(tsz * sizeof(int)));
+    RC_HIP(h->tile_k.ensure(2 * tsz * sizeof(uint32_t)));
+    RC_HIP(h->tile_v.ensure(tsz * sizeof(int)));
+    RC_HIP(h->tile_tmp.ensure(radix_tmp_ints((int)tsz) * sizeof(int)));
+    launch_tile_keys(ntiles, h->idx_user.as<int>(), h->tile_k.as<uint32_t>(), h->tile_ord.as<int>(), s);
+    radix_sort_pairs(h->tile_k.as<uint32_t>(), h->tile_ord.as<int>(), h->tile_k.as<uint32_t>() + tsz,
+                     h->tile_v.as<int>(), ntiles, std::max(1, bit_width(ns)), h->tile_tmp.as<int>(), s);
     if (h->nsys > 0) {
         // system of every position, and the positions listed system by system
         RC_HIP(h->sys_pos.ensure(nsz * sizeof(int)));
@@ -1433,6 +1445,7 @@ static SolveArgs solve_args(rc_handle_t h)
     SolveArgs a;
     a.n = h->ns;
     a.ntiles = h->ntiles;
+    a.a_tile_ord = h->tile_ord.as<int>();
     a.a_off = h->a_off.as<int>();
     a.a_width = h->a_width.as<int>();
     a.a_ind = h->a_ell_ind.as<int>();
