@@ -154,6 +154,33 @@ __device__ __forceinline__ T group_sum(T v, int G)
     return v;
 }
 
+// Thread range of a grid-stride loop over the ordered SpMV tiles in which
+// the workgroups of one XCD (workgroup b runs on XCD b % 8) share one
+// contiguous eighth of the tile order, so each XCD's L2 serves one region of
+// the input numbering.  Falls back to the plain grid stride when the grid is
+// not a multiple of 8.
+struct TileRange {
+    int first, end, stride;
+};
+__device__ __forceinline__ TileRange xcd_tile_range(int ntiles)
+{
+    constexpr int kXcds = 8;
+    TileRange r;
+    if (gridDim.x % kXcds != 0) {
+        r.first = blockIdx.x * blockDim.x + threadIdx.x;
+        r.end = ntiles * kWave;
+        r.stride = gridDim.x * blockDim.x;
+        return r;
+    }
+    const int x = blockIdx.x % kXcds, b = blockIdx.x / kXcds;
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    const int t0 = min(ntiles, x * per), t1 = min(ntiles, t0 + per);
+    r.first = t0 * kWave + b * blockDim.x + threadIdx.x;
+    r.end = t1 * kWave;
+    r.stride = (gridDim.x / kXcds) * blockDim.x;
+    return r;
+}
+
 // ---------------------------------------------------------------- K1
 // p_new = z + beta p_old (streaming), then Ap = A p_new with block partials of
 // p'Ap; the last block computes alpha.  (Forming p_new inside the SpMV gather
@@ -179,7 +206,8 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
     if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int lane = lane_id();
     double acc_dot = 0.0;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+    const TileRange tr = xcd_tile_range(a.ntiles);
+    for (int it = tr.first; it < tr.end; it += tr.stride) {
         const int tile = a.a_tile_ord[it / kWave];
         const int q = tile * kWave + lane;
         const int w = a.a_width[tile];
@@ -225,7 +253,8 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv(SolveArgs a, const double *p
 __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const double *x, double *y)
 {
     const int lane = lane_id();
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+    const TileRange tr = xcd_tile_range(a.ntiles);
+    for (int it = tr.first; it < tr.end; it += tr.stride) {
         const int tile = a.a_tile_ord[it / kWave];
         const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
@@ -620,7 +649,8 @@ __global__ void __launch_bounds__(kRedBlock) k_residual(SolveArgs a)
     PcgScalars *sc = a.sc;
     const int lane = lane_id();
     double acc = 0.0, accb = 0.0;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+    const TileRange tr = xcd_tile_range(a.ntiles);
+    for (int it = tr.first; it < tr.end; it += tr.stride) {
         const int tile = a.a_tile_ord[it / kWave];
         const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
@@ -768,7 +798,8 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, const double *
     if (sc->done) return;
     if (blockIdx.x == 0) reset_claims(a.ctr_fwd, a.ctr_bwd, threadIdx.x);
     const int lane = lane_id();
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+    const TileRange tr = xcd_tile_range(a.ntiles);
+    for (int it = tr.first; it < tr.end; it += tr.stride) {
         const int tile = a.a_tile_ord[it / kWave];
         const int q = tile * kWave + lane;
         const int w = a.a_width[tile];
@@ -795,7 +826,8 @@ __global__ void __launch_bounds__(kRedBlock) k_bspmv(SolveArgs a, const double *
 __global__ void __launch_bounds__(kRedBlock) k_bresid(SolveArgs a)
 {
     const int lane = lane_id();
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.ntiles * kWave; it += gridDim.x * blockDim.x) {
+    const TileRange tr = xcd_tile_range(a.ntiles);
+    for (int it = tr.first; it < tr.end; it += tr.stride) {
         const int tile = a.a_tile_ord[it / kWave];
         const int p = tile * kWave + lane;
         const int w = a.a_width[tile];
