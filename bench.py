@@ -7,7 +7,7 @@ rc_solve_resident (PCG to tol with the RHS already in HBM).  Both calls return
 after synchronising the library's stream; the timed loop is additionally
 bracketed by hipDeviceSynchronize.
 
-Default workload (BASELINE config 2): 3D 7-point Poisson SDDM 128^3 with
+Default workload: 3D 7-point Poisson SDDM 128^3 with
 diagonal = degree + 1e-2, converted to a Laplacian with an appended ground
 vertex, nnz-sorted ordering (seed 1), FP64 factor and PCG, one N(0,1) RHS
 (seed 42), tol 1e-6.
@@ -187,7 +187,7 @@ def main():
                     help="skip the per-structure sweep launch tuning done once before the timed steps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
