@@ -187,7 +187,7 @@ def main():
                     help="skip the per-structure sweep launch tuning done once before the timed steps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1",
+        print("bench.py measures one GPU (the library has no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
