@@ -369,6 +369,7 @@ struct MultiArgs {
     int n, K, k;                  // rows, lane stride (pow2 >= k), right-hand sides
     const int *a_ptr, *a_ind;     // CSR in position numbering
     const double *a_val;
+    const int *row_ord;           // position of every input-numbered row (the SpMV's visiting order)
     const int *l_ptr, *l_ind, *lt_ptr, *lt_ind;
     const void *l_val, *lt_val, *diag;
     int nchunks;

@@ -73,7 +73,19 @@ __global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p)
     const int q = threadIdx.x % K;
     const int rows_per_block = kMB / K;
     double acc_dot = 0.0;
-    for (int row = blockIdx.x * rows_per_block + threadIdx.x / K; row < a.n; row += gridDim.x * rows_per_block) {
+    // rows visited in the input numbering (a row's neighbours are near it
+    // there, not in positions), the workgroups of each XCD (workgroup b runs
+    // on XCD b % 8) walking one contiguous eighth of it
+    int r0 = blockIdx.x * rows_per_block + threadIdx.x / K, r1 = a.n, rstep = gridDim.x * rows_per_block;
+    if (gridDim.x % 8 == 0) {
+        const int x = blockIdx.x % 8, per = (a.n + 7) / 8;
+        const int lo = min(a.n, x * per);
+        r1 = min(a.n, lo + per);
+        r0 = lo + (blockIdx.x / 8) * rows_per_block + threadIdx.x / K;
+        rstep = (gridDim.x / 8) * rows_per_block;
+    }
+    for (int ru = r0; ru < r1; ru += rstep) {
+        const int row = a.row_ord[ru];
         double acc = 0.0;
         const int e1 = a.a_ptr[row + 1];
         int e = a.a_ptr[row];

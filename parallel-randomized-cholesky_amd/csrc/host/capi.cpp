@@ -100,6 +100,7 @@ struct rc_handle_s {
     // solve layout: level-order positions, sliced ELL of L, L^T and A
     DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP, lvl_w, cm_f, cm_b;
     DevVec tile_ord, tile_k, tile_v, tile_tmp;   // SpMV tile order (by the user index of a tile's first row)
+    DevVec pos_of_user;             // position of every user-numbered row
     DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
     DevVec a_off, a_width, a_ell_ind, a_ell_val;
     DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
@@ -1894,12 +1895,15 @@ static int multi_args(rc_handle_t h, int k, MultiArgs &a)
     RC_HIP(h->m_sc.ensure(sizeof(MultiScalars)));
     RC_HIP(h->m_part.ensure((size_t)h->reduce_blocks * 2 * K * sizeof(double)));
     RC_HIP(h->ctrs.ensure(kClaimWords * sizeof(unsigned int)));
+    RC_HIP(h->pos_of_user.ensure((n + 1) * sizeof(int)));
+    launch_inverse_perm((int)n, h->idx_user.as<int>(), h->pos_of_user.as<int>(), h->stream);
     a.n = (int)n;
     a.K = K;
     a.k = k;
     a.a_ptr = h->m_aptr.as<int>();
     a.a_ind = h->m_aind.as<int>();
     a.a_val = h->m_aval.as<double>();
+    a.row_ord = h->pos_of_user.as<int>();
     a.l_ptr = h->m_lptr.as<int>();
     a.l_ind = h->m_lind.as<int>();
     a.lt_ptr = h->m_ltptr.as<int>();
