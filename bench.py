@@ -295,6 +295,25 @@ def main():
     achieved = kern[dom]["GBps"]
 
     x = s.get_solution()
+    # mixed precision: error of the FP32-factor solution against an FP64
+    # oracle of the same system (FP64 factor and PCG to 1e-12; a direct solve
+    # of this size is out of reach), measured outside the timed steps
+    err64 = None
+    if cfg["prec"] == 32 and B is None and not batch:
+        t = time.perf_counter()
+        s64 = rc.Solver(device=0, seed=1, precision=64, maxit=3000)
+        s64.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if cfg["kind"] == "sddm" else rc.INPUT_LAPLACIAN)
+        s64.order(rc.ORDER_NNZ_SORT, 1)
+        s64.factor()
+        x_ref, st64 = s64.solve(b, tol=1e-12)
+        s64.close()
+        s.set_reference_solution(x_ref)
+        _, ste = s.solve(b, tol=args.tol)
+        err64 = {"err_A_norm": ste["err_vs_oracle"], "iterations": ste["iterations"],
+                 "oracle": (f"FP64 factor (nnz-sort) + FP64 PCG to 1e-12: {st64['iterations']} iterations, "
+                            f"true rel-res {st64['rel_res_true']:.1e}")}
+        log(f"[bench] error vs FP64 oracle: {ste['err_vs_oracle']:.3e} (A-norm, relative) "
+            f"({time.perf_counter() - t:.1f}s)")
     res = {
         "metric": "factor+PCG time to rel-res 1e-6 (s)",
         "value": float(np.mean(times)),
@@ -321,6 +340,7 @@ def main():
             "fills": fst["fills"], "nlevels": fst["nlevels"], "max_degree": fst["max_degree"],
             "kernels": kern,
             "autotune": tune,
+            "error_vs_fp64_oracle": err64,
         },
     }
     if not args.no_cpu:
