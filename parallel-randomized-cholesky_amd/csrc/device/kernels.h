@@ -188,9 +188,7 @@ constexpr int kSweepThreads = 256;
 constexpr int kTailThreads = 1024;          // 4 sweep groups
 __host__ __device__ inline int sweep_rows_pow2(int cnt)
 {
-    int p = 1;
-    while (p < cnt) p <<= 1;
-    return p;
+    return cnt <= 1 ? 1 : 1 << (32 - __builtin_clz((unsigned)(cnt - 1)));
 }
 __host__ __device__ inline int sweep_group(int cnt)
 {
@@ -208,6 +206,9 @@ void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *sli
 // position - tp0).  A tail level of cnt <= kTailMaxRows rows is one group
 // with tail_group(cnt) lanes per row: entry t of level row i lives at
 // off[L] + (t / G) * P * G + i * G + t % G (P = next power of two >= cnt).
+// One extra entry at off[ntl] (index ntail, value 0) is what lanes without a
+// slot load: the workgroup's LDS holds 0 at value slot ntail, so the masked
+// contribution is -0 * 0 and the loaded value needs no arithmetic mask.
 constexpr int kTailMaxRows = 1024;
 __host__ __device__ inline int tail_group(int cnt)
 {
@@ -215,7 +216,7 @@ __host__ __device__ inline int tail_group(int cnt)
     return g > 64 ? 64 : (g < 1 ? 1 : g);
 }
 // LDS of the tail workgroup: ntail + 1 values (16-byte padded), then one
-// int4 descriptor per tail level.
+// int4 descriptor per tail level and one empty descriptor after them.
 template <typename T>
 __host__ __device__ inline int tail_value_bytes(int ntail)
 {

@@ -510,8 +510,12 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     if (t == 0) v[nt] = (T)0;        // slot of the padding entries
     // level descriptors {start, cnt, ELL offset, slices} after the values
     const int ntl = a.ntail_levels;
+    const int ze = off[ntl];          // the zero entry after the last level
     int4 *tm = reinterpret_cast<int4 *>(reinterpret_cast<char *>(tail_lds_d) + tail_value_bytes<T>(nt));
     for (int L = t; L < ntl; L += kTailThreads) tm[L] = make_int4(a.tl_start[L], a.tl_cnt[L], off[L], slices[L]);
+    // an empty level (no rows, no slices) after the last one: the loop below
+    // solves whole groups of three levels, the ones past the end being empty
+    if (t == 0) tm[ntl] = make_int4(tp0, 0, ze, 0);
     __syncthreads();
     // Software pipeline over three register sets (A, B, C): this lane's first
     // kTailPref index/value slices of level s + 2 are loaded while level s is
@@ -519,18 +523,18 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     // is unrolled by three so that no register copy (and no wait) sits
     // between the loads and their use.
     auto fetch = [&](int s, int4 &d, int *j, T *l) {
-        s = s < ntl ? s : ntl - 1;
-        d = tm[kFwd ? s : ntl - 1 - s];
+        d = tm[s < ntl ? (kFwd ? s : ntl - 1 - s) : ntl];
         const int stride = sweep_rows_pow2(d.y) * tail_group(d.y);
 #pragma unroll
         for (int u = 0; u < kTailPref; ++u) {
-            // every lane loads (entry 0 when it has no slice u; its index is a
-            // valid LDS slot) and the value is masked arithmetically, so the
-            // loads are never predicated away
+            // every lane loads (the zero entry when it has no slice u), so the
+            // loads are never predicated away, and nothing is computed from
+            // the loaded values before their level is solved (a mask applied
+            // here would make the compiler wait for these loads at once)
             const bool has = t < stride && u < d.w;
-            const int e = has ? d.z + t + u * stride : 0;
+            const int e = has ? d.z + t + u * stride : ze;
             j[u] = ind[e];
-            l[u] = val[e] * (has ? (T)1 : (T)0);
+            l[u] = val[e];
         }
     };
     auto solve = [&](const int4 &d, const int *j, const T *l) {
@@ -544,17 +548,17 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
         if (t < stride) {
             // slices past the prefetched ones (long rows), eight loads in
-            // flight at a time; a masked slot adds -0 * v = -0, which leaves
-            // acc unchanged
+            // flight at a time; a masked slot (the zero entry) adds
+            // -0 * 0 = -0, which leaves acc unchanged
             for (int sl = kTailPref; sl < d.w; sl += 8) {
                 int jj[8];
                 T ll[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const bool has = sl + u < d.w;
-                    const int e = has ? d.z + t + (sl + u) * stride : 0;
+                    const int e = has ? d.z + t + (sl + u) * stride : ze;
                     jj[u] = ind[e];
-                    ll[u] = val[e] * (has ? (T)1 : (T)0);
+                    ll[u] = val[e];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
@@ -572,13 +576,13 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         fetch(0, dA, jA, lA);
         fetch(1, dB, jB, lB);
     }
+    // no exits inside the group: an exit path would carry the newest loads
+    // to the loop header, where the compiler then waits for them every level
     for (int s = 0; s < ntl; s += 3) {
         fetch(s + 2, dC, jC, lC);
         solve(dA, jA, lA);
-        if (s + 1 >= ntl) break;
         fetch(s + 3, dA, jA, lA);
         solve(dB, jB, lB);
-        if (s + 2 >= ntl) break;
         fetch(s + 4, dB, jB, lB);
         solve(dC, jC, lC);
     }

@@ -1019,7 +1019,8 @@ static int assemble(rc_handle_t h, double *ms_out)
         int64_t rows = 0;
         while (tl > 0) {
             const int c = hl[(size_t)tl] - hl[(size_t)tl - 1];
-            const int64_t bytes = ((rows + c + 1) * (int64_t)sizeof(T) + 15) / 16 * 16 + 16 * (int64_t)(nl - tl + 1);
+            // values + one descriptor per tail level + the empty descriptor
+            const int64_t bytes = ((rows + c + 1) * (int64_t)sizeof(T) + 15) / 16 * 16 + 16 * (int64_t)(nl - tl + 2);
             if (c >= h->tail_max_rows || bytes > (int64_t)h->tail_lds_cap) break;
             rows += c;
             --tl;
@@ -1084,7 +1085,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     h->ntail_levels = nl - h->tail_level;
     h->tail_pos0 = nl > 0 ? hl[(size_t)h->tail_level] : ns;
     h->ntail = ns - h->tail_pos0;
-    h->tail_lds_bytes = tail_value_bytes<T>(h->ntail) + 16 * h->ntail_levels;
+    h->tail_lds_bytes = tail_value_bytes<T>(h->ntail) + 16 * (h->ntail_levels + 1);
     if (h->ntail > 0) {
         const int ntl = h->ntail_levels, tp0 = h->tail_pos0, nt = h->ntail;
         std::vector<int> ls((size_t)ntl), lc((size_t)ntl);
@@ -1137,11 +1138,15 @@ static int assemble(rc_handle_t h, double *ms_out)
             int total = 0;
             RC_HIP(hipMemcpyAsync(&total, off.as<int>() + ntl, sizeof(int), hipMemcpyDeviceToHost, s));
             RC_HIP(hipStreamSynchronize(s));
-            RC_HIP(ind.ensure((size_t)std::max(total, 1) * sizeof(int)));
-            RC_HIP(val.ensure((size_t)std::max(total, 1) * sizeof(T)));
+            // one more entry after the levels: the zero slot masked lanes load
+            RC_HIP(ind.ensure(((size_t)total + 1) * sizeof(int)));
+            RC_HIP(val.ensure(((size_t)total + 1) * sizeof(T)));
             launch_tail_fill<T>(ntl, h->tl_start.as<int>(), h->tl_cnt.as<int>(), tp0, h->ti_ptr.as<int>(),
                                 h->ti_ind.as<int>(), h->ti_val.as<T>(), off.as<int>(), slices.as<int>(), nt,
                                 ind.as<int>(), val.as<T>(), s);
+            const T zero_val = (T)0;
+            RC_HIP(hipMemcpyAsync(ind.as<int>() + total, &nt, sizeof(int), hipMemcpyHostToDevice, s));
+            RC_HIP(hipMemcpyAsync(val.as<T>() + total, &zero_val, sizeof(T), hipMemcpyHostToDevice, s));
             RC_HIP(hipStreamSynchronize(s));
             return RC_OK;
         };
