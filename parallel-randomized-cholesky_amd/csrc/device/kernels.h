@@ -279,6 +279,9 @@ struct SolveArgs {
     const int *lf_off, *lf_width, *lf_ind;
     const int *lb_off, *lb_width, *lb_ind;
     const void *lf_val, *lb_val, *diag;
+    // entry after the last chunk of each sweep's ELL: index n, value 0; what
+    // lanes without a slot load when a chunk's slices are prefetched
+    int lf_zero, lb_zero;
     // vectors in position numbering, n + 1 entries
     double *x, *r, *ap, *p0, *p1;
     void *y, *z;                  // precision of the factor, sentinel protocol

@@ -123,6 +123,30 @@ __device__ __forceinline__ T grouped_row_solve(const int *ind, const T *val, int
     return acc;
 }
 
+// The same sum when the row's first kBulkPref slices are already in
+// registers (jp/lp, loaded a chunk earlier; a lane without slice u holds the
+// zero entry, whose value is 0 and whose index is the dummy slot).  Slices
+// past them are loaded here as above.
+constexpr int kBulkPref = 8;
+template <typename T>
+__device__ __forceinline__ T grouped_row_solve_pref(const int *jp, const T *lp, const int *ind, const T *val, int nsl,
+                                                    int stride, T *vec, T acc, long long spin, PcgScalars *sc,
+                                                    int nap_max)
+{
+    T v[kBulkPref];
+#pragma unroll
+    for (int u = 0; u < kBulkPref; ++u) v[u] = vec[jp[u]];
+#pragma unroll
+    for (int u = 0; u < kBulkPref; ++u) {
+        if (is_sent(v[u])) v[u] = wait_slow(vec + jp[u], spin, sc, nap_max);
+        acc = fma_t(-lp[u], v[u], acc);
+    }
+    if (nsl > kBulkPref)
+        acc = grouped_row_solve<T>(ind + kBulkPref * stride, val + kBulkPref * stride, nsl - kBulkPref, stride, vec,
+                                   acc, spin, sc, nap_max);
+    return acc;
+}
+
 // Sum over the G lanes of a row group (G a power of two <= 64, groups aligned
 // inside the wave), valid in the group's LAST lane: a shift-and-add scan over
 // DPP row shifts (1, 2, 4, 8 lanes inside 16-lane rows) and row broadcasts
@@ -284,7 +308,7 @@ __global__ void __launch_bounds__(kRedBlock) k_spmv_plain(SolveArgs a, const dou
 // the dependencies.
 template <typename T>
 __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, double alpha, const double *p_cur,
-                                          int update_rx)
+                                          int update_rx, const int *jp, const T *lp)
 {
     const T *lval = (const T *)a.lf_val;
     T *y = (T *)a.y;
@@ -309,7 +333,8 @@ __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, dou
     }
     if (t < stride) {
         const int off = m.z + t;
-        acc = grouped_row_solve<T>(a.lf_ind + off, lval + off, m.w, stride, y, acc, a.spin_ticks, a.sc, a.nap_max);
+        acc = grouped_row_solve_pref<T>(jp, lp, a.lf_ind + off, lval + off, m.w, stride, y, acc, a.spin_ticks, a.sc,
+                                        a.nap_max);
     }
     if (lead) {
         if (update_rx) {
@@ -327,7 +352,7 @@ __device__ __forceinline__ void fwd_chunk(const SolveArgs &a, int4 m, int t, dou
 }
 
 template <typename T>
-__device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
+__device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t, const int *jp, const T *lp)
 {
     const T *ltval = (const T *)a.lb_val;
     const T *diag = (const T *)a.diag;
@@ -347,7 +372,8 @@ __device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
     }
     if (t < stride) {
         const int off = m.z + t;
-        acc = grouped_row_solve<T>(a.lb_ind + off, ltval + off, m.w, stride, z, acc, a.spin_ticks, a.sc, a.nap_max);
+        acc = grouped_row_solve_pref<T>(jp, lp, a.lb_ind + off, ltval + off, m.w, stride, z, acc, a.spin_ticks, a.sc,
+                                        a.nap_max);
     }
     if (lead) {
         y[k] = sent_value(T());
@@ -361,14 +387,16 @@ __device__ __forceinline__ void bwd_chunk(const SolveArgs &a, int4 m, int t)
 // Persistent part: workgroups claim chunks of the wide levels in order
 // (forward) or reverse order (backward); dependencies are awaited per value.
 // Claims run ahead of the work: while chunk i is processed, the workgroup
-// already knows chunk i+1 (every thread loads its descriptor in the
-// meantime) and thread 0 has the claim of chunk i+2 in flight, so no round
-// trip is exposed between chunks.  Holding claims ahead cannot deadlock: the
-// smallest unfinished claimed chunk is always some workgroup's current chunk,
-// and its dependencies are all earlier chunks.
+// already holds chunk i+1's descriptor and its first kBulkPref index/value
+// slices (loaded before chunk i's dependency gathers, so both round trips
+// overlap), loads chunk i+2's descriptor, and thread 0 has the claim of chunk
+// i+3 in flight, so no round trip is exposed between chunks.  Holding claims
+// ahead cannot deadlock: the smallest unfinished claimed chunk is always some
+// workgroup's current chunk, and its dependencies are all earlier chunks.
 template <typename T, bool kFwd>
 __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const double *p_cur, int update_rx)
 {
+    __shared__ int s_init[3];
     __shared__ int s_claim[2];
     __shared__ int s_stop;
     PcgScalars *sc = a.sc;
@@ -379,6 +407,9 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
     const int q = (int)(blockIdx.x % (unsigned int)nq);
     unsigned int *ctr = (kFwd ? a.ctr_fwd : a.ctr_bwd) + q * kClaimStride;
     const int4 *meta = kFwd ? a.cm_f : a.cm_b;
+    const int *ind = kFwd ? a.lf_ind : a.lb_ind;
+    const T *val = (const T *)(kFwd ? a.lf_val : a.lb_val);
+    const int ze = kFwd ? a.lf_zero : a.lb_zero;
     const int nb = a.bulk_chunks;
     const int t = threadIdx.x;
     // descriptor of claim c (clamped: a claim past the end is never processed).
@@ -393,25 +424,48 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
         return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
                          __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
     };
+    // this lane's first kBulkPref slices of chunk m; every lane loads (the
+    // zero entry where it has no slice), and nothing is computed from the
+    // values until the chunk is processed
+    auto fetch = [&](const int4 &m, int *j, T *l) {
+        const int stride = sweep_rows_pow2(m.y) * sweep_group(m.y);
+#pragma unroll
+        for (int u = 0; u < kBulkPref; ++u) {
+            const bool has = t < stride && u < m.w;
+            const int e = has ? m.z + t + u * stride : ze;
+            j[u] = ind[e];
+            l[u] = val[e];
+        }
+    };
     int c_pend = 0;
     unsigned int st_pend = 0;
     if (t == 0) {
-        s_claim[0] = (int)atomicAdd(ctr, 1u) * nq + q;
-        s_claim[1] = (int)atomicAdd(ctr, 1u) * nq + q;
+        s_init[0] = (int)atomicAdd(ctr, 1u) * nq + q;
+        s_init[1] = (int)atomicAdd(ctr, 1u) * nq + q;
+        s_init[2] = (int)atomicAdd(ctr, 1u) * nq + q;
         c_pend = (int)atomicAdd(ctr, 1u) * nq + q;
         st_pend = ld_relaxed(&sc->status);
     }
     __syncthreads();
-    int c_cur = s_claim[0], c_nxt = s_claim[1];
+    int c_cur = s_init[0], c_nxt = s_init[1], c_nxt2 = s_init[2];
     int4 m_cur = uniform(desc(c_cur));
+    int4 m_nxt = uniform(desc(c_nxt));
+    int j_cur[kBulkPref], j_nxt[kBulkPref];
+    T l_cur[kBulkPref], l_nxt[kBulkPref];
+    fetch(m_cur, j_cur, l_cur);
+    // the first chunk's slices are waited for here, once: were they still in
+    // flight at the loop entry, the compiler would wait at the top of every
+    // iteration for loads issued that early, i.e. for the prefetch as well
+    __builtin_amdgcn_s_waitcnt(0);
     int par = 0;
     while (c_cur < nb) {
-        const int4 m_nxt = desc(c_nxt);
-        if (kFwd) fwd_chunk<T>(a, m_cur, t, alpha, p_cur, update_rx);
-        else bwd_chunk<T>(a, m_cur, t);
+        fetch(m_nxt, j_nxt, l_nxt);
+        const int4 m_nxt2 = desc(c_nxt2);
+        if (kFwd) fwd_chunk<T>(a, m_cur, t, alpha, p_cur, update_rx, j_cur, l_cur);
+        else bwd_chunk<T>(a, m_cur, t, j_cur, l_cur);
         // made uniform before thread 0 issues new loads, so that waiting for
         // it does not also wait for them
-        const int4 m_next_u = uniform(m_nxt);
+        const int4 m_nxt2_u = uniform(m_nxt2);
         if (t == 0) {
             // publish the claim taken a chunk ago, take the next one; both
             // this atomic and the status load complete during the next chunk
@@ -423,8 +477,15 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
         __syncthreads();
         if (s_stop & kPcgTimeout) break;
         c_cur = c_nxt;
-        c_nxt = s_claim[par];
-        m_cur = m_next_u;
+        c_nxt = c_nxt2;
+        c_nxt2 = s_claim[par];
+        m_cur = m_nxt;
+        m_nxt = m_nxt2_u;
+#pragma unroll
+        for (int u = 0; u < kBulkPref; ++u) {
+            j_cur[u] = j_nxt[u];
+            l_cur[u] = l_nxt[u];
+        }
         par ^= 1;
     }
 }

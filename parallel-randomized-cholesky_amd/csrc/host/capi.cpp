@@ -1060,10 +1060,20 @@ static int assemble(rc_handle_t h, double *ms_out)
     if (rc != RC_OK) return rc;
     rc = sweep_offsets(h->lt_ptr.as<int>(), h->lb_width.as<int>(), h->lb_off.as<int>(), h->ell_u);
     if (rc != RC_OK) return rc;
-    RC_HIP(h->lf_ind.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(int)));
-    RC_HIP(h->lf_val.ensure((size_t)std::max<int64_t>(h->ell_l, 1) * sizeof(T)));
-    RC_HIP(h->lb_ind.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(int)));
-    RC_HIP(h->lb_val.ensure((size_t)std::max<int64_t>(h->ell_u, 1) * sizeof(T)));
+    // one more entry after the chunks: the zero slot (index ns, value 0)
+    // that lanes without a slot load when the sweeps prefetch a chunk
+    RC_HIP(h->lf_ind.ensure(((size_t)h->ell_l + 1) * sizeof(int)));
+    RC_HIP(h->lf_val.ensure(((size_t)h->ell_l + 1) * sizeof(T)));
+    RC_HIP(h->lb_ind.ensure(((size_t)h->ell_u + 1) * sizeof(int)));
+    RC_HIP(h->lb_val.ensure(((size_t)h->ell_u + 1) * sizeof(T)));
+    {
+        const T zero_val = (T)0;
+        RC_HIP(hipMemcpyAsync(h->lf_ind.as<int>() + h->ell_l, &ns, sizeof(int), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->lf_val.as<T>() + h->ell_l, &zero_val, sizeof(T), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->lb_ind.as<int>() + h->ell_u, &ns, sizeof(int), hipMemcpyHostToDevice, s));
+        RC_HIP(hipMemcpyAsync(h->lb_val.as<T>() + h->ell_u, &zero_val, sizeof(T), hipMemcpyHostToDevice, s));
+        RC_HIP(hipStreamSynchronize(s));
+    }
     EllSrc fs{h->l_ptr.as<int>(), nullptr, h->lt_val.p, h->vals0.as<int>(), h->rowk.as<int>(), nullptr};
     launch_sweep_fill<T, T>(nchunks, h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), h->order.as<int>(), fs,
                             h->lf_off.as<int>(), h->lf_width.as<int>(), ns, (T)0, h->lf_ind.as<int>(), h->lf_val.as<T>(),
@@ -1464,6 +1474,8 @@ static SolveArgs solve_args(rc_handle_t h)
     a.lf_off = h->lf_off.as<int>();
     a.lf_width = h->lf_width.as<int>();
     a.lf_ind = h->lf_ind.as<int>();
+    a.lf_zero = (int)h->ell_l;
+    a.lb_zero = (int)h->ell_u;
     a.lb_off = h->lb_off.as<int>();
     a.lb_width = h->lb_width.as<int>();
     a.lb_ind = h->lb_ind.as<int>();
