@@ -1567,6 +1567,15 @@ static int alloc_solve(rc_handle_t h)
     RC_HIP(h->partials.ensure((size_t)h->reduce_blocks * 2 * sizeof(double)));
     RC_HIP(h->ctrs.ensure(kClaimWords * sizeof(unsigned int)));
     if ((size_t)h->y.p != old_y) invalidate_graph(h);
+    // the dummy slot n of every vector holds 0 from here on: ELL padding and
+    // the zero entries that masked lanes load point at it, also when the
+    // sweeps run before any PCG (timed or tuned first) on memory whose last
+    // user left a sentinel there
+    const size_t ns = (size_t)h->ns;
+    for (DevVec *v : {&h->x, &h->r, &h->ap, &h->p0, &h->p1, &h->b})
+        RC_HIP(hipMemsetAsync((char *)v->p + ns * 8, 0, 8, h->stream));
+    RC_HIP(hipMemsetAsync((char *)h->y.p + ns * s, 0, s, h->stream));
+    RC_HIP(hipMemsetAsync((char *)h->z.p + ns * s, 0, s, h->stream));
     return RC_OK;
 }
 
@@ -2077,6 +2086,10 @@ int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, d
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
     if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and solve once before timing");
+    {
+        const int rc_alloc = alloc_solve(h);     // sizes the vectors, zeroes their dummy slot
+        if (rc_alloc != RC_OK) return rc_alloc;
+    }
     if (reps <= 0 || !ms_per_launch) return fail(h, RC_ERR_ARG, "bad timing arguments");
     RC_HIP(hipSetDevice(h->opt.device));
     hipStream_t s = h->stream;
