@@ -26,7 +26,7 @@ ERR_ARG, ERR_IO, ERR_FORMAT, ERR_NOT_DOMINANT, ERR_STATE, ERR_HIP, ERR_POOL = -1
 ERR_TIMEOUT, ERR_BREAKDOWN, ERR_NO_DEVICE, ERR_NOMEM, ERR_NOT_SYMMETRIC, ERR_MAXIT = -8, -9, -10, -11, -12, -13
 
 INPUT_LAPLACIAN, INPUT_GROUNDED, INPUT_SDDM = 0, 1, 2
-ORDER_AS_IS, ORDER_NNZ_SORT, ORDER_RANDOM, ORDER_AMD, ORDER_LEARNED = 0, 1, 2, 3, 4
+ORDER_AS_IS, ORDER_NNZ_SORT, ORDER_RANDOM, ORDER_AMD, ORDER_LEARNED, ORDER_ND = 0, 1, 2, 3, 4, 5
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 RHS_ZERO_SUM_UNIFORM, RHS_NORMAL, RHS_NORMAL_PROJECTED = 0, 1, 2
 KERNEL_SPMV, KERNEL_TRSV_FWD, KERNEL_TRSV_BWD, KERNEL_PCG_ITER, KERNEL_COPY = 0, 1, 2, 3, 4
@@ -69,6 +69,7 @@ class Options(ctypes.Structure):
         ("maxit", ctypes.c_int),
         ("pool_factor", ctypes.c_double),
         ("verbose", ctypes.c_int),
+        ("host_threads", ctypes.c_int),
     ]
 
 
@@ -162,13 +163,13 @@ class Solver:
     device=-1 makes a host-only handle (input, ordering, RHS; no factor/solve)."""
 
     def __init__(self, device=0, seed=1, num_blocks=0, precision=64, stop_rule=STOP_REFERENCE, maxit=300,
-                 pool_factor=4.0, verbose=0):
+                 pool_factor=4.0, verbose=0, host_threads=0):
         self.lib = load_library()
         o = Options()
         self.lib.rc_default_options(ctypes.byref(o))
         o.device, o.seed, o.num_blocks = device, seed, num_blocks
         o.factor_precision, o.stop_rule, o.maxit = precision, stop_rule, maxit
-        o.pool_factor, o.verbose = pool_factor, verbose
+        o.pool_factor, o.verbose, o.host_threads = pool_factor, verbose, host_threads
         h = ctypes.c_void_p()
         st = self.lib.rc_create(ctypes.byref(h), ctypes.byref(o))
         if st != OK:

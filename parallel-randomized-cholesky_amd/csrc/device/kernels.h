@@ -363,6 +363,7 @@ void gnn_backward(const GnnArgs &g, const float *w, const float *ds, float *dw, 
 constexpr int kMaxRhs = 64;
 struct MultiScalars {
     double rz[kMaxRhs], rz0[kMaxRhs], alpha[kMaxRhs], beta[kMaxRhs], bb[kMaxRhs], res2[kMaxRhs];
+    double rr[kMaxRhs];           // r'r of the last dot pass (true-residual stop rule)
     int iter[kMaxRhs], done[kMaxRhs], breakdown[kMaxRhs];
     double tol;
     int maxit, stop_rule, status, all_done;

@@ -316,6 +316,7 @@ __global__ void __launch_bounds__(kMB) k_mdot(MultiArgs a, int init)
             rr += ld_relaxed_val(a.partials + (size_t)(gridDim.x + b) * K + qq);
         }
         if (qq < a.k) {
+            if (need_rr && !(!init && sc->done[qq])) sc->rr[qq] = rr;
             if (init) {
                 sc->rz0[qq] = rz;
                 sc->rz[qq] = rz;

@@ -12,175 +12,21 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
+#include <stdexcept>
 #include <random>
 #include <string>
 #include <vector>
 
 #include "../../../include/rchol.h"
 #include "../device/kernels.h"
+#include "handle.hpp"
 #include "mmio.hpp"
 #include "philox.h"
 #include "sparse.hpp"
 #include "gnn_host.hpp"
 
 using namespace rc;
-
-namespace {
-
-struct DevVec {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevVec() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    // grow-only; returns hipSuccess or the allocation error
-    hipError_t ensure(size_t b)
-    {
-        if (b <= bytes && p) return hipSuccess;
-        release();
-        if (b == 0) b = 8;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) { p = nullptr; return e; }
-        bytes = b;
-        return hipSuccess;
-    }
-    template <typename T> T *as() const { return static_cast<T *>(p); }
-};
-
-int64_t pow2_at_least(int64_t v)
-{
-    int64_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-}  // namespace
-
-struct rc_handle_s {
-    rc_options opt{};
-    std::string err;
-    hipStream_t stream = nullptr;
-    long long spin_ticks = 0;
-    int num_blocks = 0;
-    bool host_only = false;
-
-    // loaded system
-    int kind = -1;
-    Csc base;
-    int n_user = 0;
-    std::vector<int> perm;          // perm[new] = old (user numbering)
-    std::vector<float> gnn_w;
-
-    // prepared system
-    bool host_ready = false;
-    bool prepared = false;
-    Csc fac;                        // permuted Laplacian that is factored
-    Csc sys;                        // permuted PCG system
-    bool trim = false;
-    int nf = 0, ns = 0;
-    int64_t nnz_low = 0;
-
-    // factor inputs / working set
-    DevVec lo_ptr, lo_row, lo_w, dep_init;
-    DevVec big_list, big_count;
-    DevVec dep, head, level, list0, list1, seg_count, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
-        out_cnt, diag64, scratch;
-    long long pool_cap = 0, out_cap = 0;
-    int scratch_cap = 0;
-    int rounds = 0;
-
-    // factor result: CSR of L^T in the factored numbering (export / dump)
-    bool factored = false;
-    int prec = 64;
-    DevVec lt_ptr, lt_ind, lt_val, diagT;
-    // solve layout: level-order positions, sliced ELL of L, L^T and A
-    DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP, lvl_w, cm_f, cm_b;
-    DevVec tile_ord, tile_k, tile_v, tile_tmp;   // SpMV tile order (by the user index of a tile's first row)
-    DevVec pos_of_user;             // position of every user-numbered row
-    DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
-    DevVec a_off, a_width, a_ell_ind, a_ell_val;
-    DevVec tmp_a, tmp_b, tmp_scan, keys0, keys1, vals0, vals1, rowk, l_ptr;
-    int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
-    int nlevels = 0, nchunks = 0, ntiles = 0;
-    int tail_level = 0, bulk_chunks = 0;
-    int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets
-    // tail of the level order (LDS-resident solve)
-    int tail_max_rows = 256, tail_lds_cap = -1;
-    int nap_max = 1;                // measured best: poll again after one s_sleep 2
-    int seg_cap_limit = 0;          // > 0: ready-list segments hold at most this many entries (exercises the overflow region)
-    int tail_pos0 = 0, ntail = 0, ntail_levels = 0, tail_lds_bytes = 0;
-    DevVec tl_start, tl_cnt, tp_ptr, tp_ind, tp_val, tf_off, tf_slices, tf_ind, tf_val, tb_off, tb_slices, tb_ind,
-        tb_val, ti_ptr, ti_ind, ti_val, tacc;
-    rc_factor_stats fstats{};
-
-    // PCG
-    DevVec a_ptr, a_ind, a_val;
-    DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d;
-    DevVec calib;                   // copy-kernel calibration buffer (rc_time_kernel)
-    int reduce_blocks = 1024;
-    int sweep_blocks = 1024;
-    int sweep_blocks_max = 1024;    // occupancy limit of the persistent sweep grid
-    bool rhs_set = false;
-    std::vector<double> xref;
-    hipGraphExec_t graph = nullptr;
-    int graph_prec = 0;
-
-    // ordering network: graph + activations of the loaded system
-    bool gnn_ready = false, gnn_keep = false;
-    int gnn_n = 0;
-    DevVec g_ptr, g_ind, g_a, g_at, g_H0, g_M1, g_H[3], g_M[3], g_Z[3], g_dA, g_dB, g_dC, g_part, g_w, g_dw, g_s, g_ds;
-
-    // batched systems (nsys == 0: one system)
-    int nsys = 0;
-    std::vector<int64_t> sys_off;   // nsys + 1, user numbering
-    std::vector<int> sys_user;      // user row -> system
-    int nblk = 0;
-    DevVec sys_user_d, sys_fac, sys_pos, spos, blk_tab, bt_state;
-    std::vector<rc_solve_stats> bstats;
-
-    // multi-RHS: CSRs in position numbering (built lazily per factor) + vectors
-    bool multi_ready = false;
-    DevVec m_aptr, m_aind, m_aval, m_lptr, m_lind, m_lval, m_ltptr, m_ltind, m_ltval;
-    DevVec m_x, m_r, m_ap, m_p0, m_p1, m_y, m_z, m_b, m_B, m_X, m_sc, m_part;
-    int m_k = 0;                    // right-hand sides held in m_B (user order, column-major)
-    bool m_solved = false;
-
-    ~rc_handle_s()
-    {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-#define RC_HIP(expr)                                                                           \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) {                                                                \
-            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
-            return e_ == hipErrorOutOfMemory ? RC_ERR_NOMEM : RC_ERR_HIP;                      \
-        }                                                                                      \
-    } while (0)
-
-#define RC_CHECK_HANDLE()                      \
-    do {                                       \
-        if (!h) return RC_ERR_ARG;             \
-    } while (0)
-
-static int fail(rc_handle_t h, int code, const std::string &msg)
-{
-    h->err = msg;
-    return code;
-}
-
-static void invalidate_graph(rc_handle_t h)
-{
-    if (h->graph) (void)hipGraphExecDestroy(h->graph);
-    h->graph = nullptr;
-}
 
 extern "C" {
 
@@ -219,7 +65,7 @@ void rc_default_options(rc_options *o)
     o->verbose = 0;
 }
 
-int rc_create(rc_handle_t *out, const rc_options *opt)
+static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
 {
     if (!out) return RC_ERR_ARG;
     *out = nullptr;
@@ -227,7 +73,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     rc_default_options(&o);
     if (opt) o = *opt;
     if (o.factor_precision != 32 && o.factor_precision != 64) return RC_ERR_ARG;
-    if (o.maxit < 0 || o.pool_factor <= 0.0) return RC_ERR_ARG;
+    if (o.maxit < 0 || o.pool_factor <= 0.0 || o.host_threads < 0) return RC_ERR_ARG;
     if (o.device == -1) {                           // host-only handle: input, ordering, RHS
         rc_handle_t h = new (std::nothrow) rc_handle_s();
         if (!h) return RC_ERR_NOMEM;
@@ -270,7 +116,7 @@ int rc_create(rc_handle_t *out, const rc_options *opt)
     return RC_OK;
 }
 
-int rc_destroy(rc_handle_t h)
+static int rc_destroy_impl(rc_handle_t h)
 {
     if (!h) return RC_ERR_ARG;
     if (!h->host_only) (void)hipSetDevice(h->opt.device);
@@ -299,10 +145,21 @@ static void reset_system(rc_handle_t h)
     h->sys_off.clear();
     h->sys_user.clear();
     h->bstats.clear();
+    // multi-RHS state belongs to the previous system: its vectors were sized
+    // for that system's n
+    h->multi_ready = false;
+    h->m_k = 0;
+    h->m_n = 0;
+    h->m_solved = false;
+    // pools are sized from the new system's nnz at its first factor (the
+    // device buffers themselves are grow-only and stay allocated)
+    h->pool_cap = 0;
+    h->out_cap = 0;
+    h->scratch_cap = 0;
     invalidate_graph(h);
 }
 
-int rc_set_triplets(rc_handle_t h, int n, int64_t m, const int *rows, const int *cols, const double *vals, int kind)
+static int rc_set_triplets_impl(rc_handle_t h, int n, int64_t m, const int *rows, const int *cols, const double *vals, int kind)
 {
     RC_CHECK_HANDLE();
     if (n <= 0 || m < 0 || (m > 0 && (!rows || !cols || !vals))) return fail(h, RC_ERR_ARG, "bad triplet arrays");
@@ -320,7 +177,7 @@ int rc_set_triplets(rc_handle_t h, int n, int64_t m, const int *rows, const int 
     return RC_OK;
 }
 
-int rc_set_batch(rc_handle_t h, int nsys, const int *sys_n, const int64_t *sys_m, const int *rows, const int *cols,
+static int rc_set_batch_impl(rc_handle_t h, int nsys, const int *sys_n, const int64_t *sys_m, const int *rows, const int *cols,
                  const double *vals)
 {
     RC_CHECK_HANDLE();
@@ -357,7 +214,7 @@ int rc_set_batch(rc_handle_t h, int nsys, const int *sys_n, const int64_t *sys_m
     return RC_OK;
 }
 
-int rc_get_batch(rc_handle_t h, int *nsys, int64_t *offsets)
+static int rc_get_batch_impl(rc_handle_t h, int *nsys, int64_t *offsets)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
@@ -369,18 +226,18 @@ int rc_get_batch(rc_handle_t h, int *nsys, int64_t *offsets)
     return RC_OK;
 }
 
-int rc_load_mtx(rc_handle_t h, const char *path, int kind)
+static int rc_load_mtx_impl(rc_handle_t h, const char *path, int kind)
 {
     RC_CHECK_HANDLE();
     if (!path) return fail(h, RC_ERR_ARG, "null path");
     MtxTriplets t;
-    int rc = read_mtx(path, t, h->err);
+    int rc = read_mtx(path, t, h->err, h->opt.host_threads);
     if (rc != RC_OK) return rc;
     if (t.nrows != t.ncols) return fail(h, RC_ERR_FORMAT, "matrix is not square");   // pre_process.hpp:358
     return rc_set_triplets(h, t.nrows, (int64_t)t.rows.size(), t.rows.data(), t.cols.data(), t.vals.data(), kind);
 }
 
-int rc_get_size(rc_handle_t h, int64_t *n, int64_t *nnz)
+static int rc_get_size_impl(rc_handle_t h, int64_t *n, int64_t *nnz)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
@@ -398,7 +255,7 @@ int rc_get_size(rc_handle_t h, int64_t *n, int64_t *nnz)
 }
 
 // ------------------------------------------------------------------ ordering
-int rc_order(rc_handle_t h, int method, uint64_t seed)
+static int rc_order_impl(rc_handle_t h, int method, uint64_t seed)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
@@ -416,6 +273,7 @@ int rc_order(rc_handle_t h, int method, uint64_t seed)
         p = amd_perm(h->base, nv);
         if ((int)p.size() != nv) return fail(h, RC_ERR_ARG, "AMD ordering failed");
         break;
+    case RC_ORDER_ND: p = nd_perm(h->base, nv, seed, h->opt.host_threads); break;
     case RC_ORDER_LEARNED:
         if (h->gnn_w.empty() || h->host_only) p = nnz_sort_perm(h->base, nv, seed, skip);      // fallback
         else {
@@ -436,7 +294,7 @@ int rc_order(rc_handle_t h, int method, uint64_t seed)
     return RC_OK;
 }
 
-int rc_get_permutation(rc_handle_t h, int *perm)
+static int rc_get_permutation_impl(rc_handle_t h, int *perm)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0 || !perm) return fail(h, RC_ERR_ARG, "no matrix or null array");
@@ -444,7 +302,7 @@ int rc_get_permutation(rc_handle_t h, int *perm)
     return RC_OK;
 }
 
-int rc_set_gnn_weights(rc_handle_t h, const float *w, int64_t count)
+static int rc_set_gnn_weights_impl(rc_handle_t h, const float *w, int64_t count)
 {
     RC_CHECK_HANDLE();
     if (count < 0 || (count > 0 && !w)) return fail(h, RC_ERR_ARG, "bad weights");
@@ -453,7 +311,7 @@ int rc_set_gnn_weights(rc_handle_t h, const float *w, int64_t count)
     return RC_OK;
 }
 
-int rc_set_permutation(rc_handle_t h, const int *perm)
+static int rc_set_permutation_impl(rc_handle_t h, const int *perm)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
@@ -541,7 +399,7 @@ static GnnArgs gnn_args(rc_handle_t h)
     return g;
 }
 
-int rc_gnn_scores(rc_handle_t h, const float *w, float *scores)
+static int rc_gnn_scores_impl(rc_handle_t h, const float *w, float *scores)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -560,7 +418,7 @@ int rc_gnn_scores(rc_handle_t h, const float *w, float *scores)
     return RC_OK;
 }
 
-int rc_gnn_gradient(rc_handle_t h, const float *w, const float *dscores, float *dw)
+static int rc_gnn_gradient_impl(rc_handle_t h, const float *w, const float *dscores, float *dw)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -700,7 +558,7 @@ static int prepare(rc_handle_t h)
     return RC_OK;
 }
 
-int rc_get_prepared_csc(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *colptr, int *rowind, double *val)
+static int rc_get_prepared_csc_impl(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *colptr, int *rowind, double *val)
 {
     RC_CHECK_HANDLE();
     int rc = prepare_host(h);
@@ -713,7 +571,7 @@ int rc_get_prepared_csc(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *colptr
     return RC_OK;
 }
 
-int rc_write_prepared_mtx(rc_handle_t h, const char *path)
+static int rc_write_prepared_mtx_impl(rc_handle_t h, const char *path)
 {
     RC_CHECK_HANDLE();
     if (!path) return fail(h, RC_ERR_ARG, "null path");
@@ -786,11 +644,14 @@ static FactorArgs factor_args(rc_handle_t h)
 static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
 {
     const long long slack = factor_pool_slack(h->num_blocks);
-    if (h->pool_cap == 0) {
-        h->pool_cap = std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low)) + slack;
-        h->out_cap = h->pool_cap + h->nnz_low + slack;
-        h->scratch_cap = 8192;
-    }
+    // fill records are addressed by int32 indices (FillRec.next, list heads),
+    // so the fill pool holds at most kMaxPoolRecords records
+    const long long want =
+        std::max<long long>(1024, (long long)std::ceil(h->opt.pool_factor * (double)h->nnz_low)) + slack;
+    if (h->pool_cap < want) h->pool_cap = want;
+    if (h->pool_cap > kMaxPoolRecords) h->pool_cap = kMaxPoolRecords;
+    h->out_cap = std::max(h->out_cap, h->pool_cap + h->nnz_low + slack);
+    if (h->scratch_cap < 8192) h->scratch_cap = 8192;
     *regrowths = 0;
     hipStream_t s = h->stream;
     for (int attempt = 0; attempt < 8; ++attempt) {
@@ -856,6 +717,8 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
                 return fail(h, RC_ERR_STATE, "elimination did not retire every column (" + std::to_string(done) + "/" +
                                                  std::to_string(h->nf - 1) + ")");
             h->rounds = used;
+            if (c[kCtrNnzOff] > (unsigned long long)kMaxFactorEntries)
+                return fail(h, RC_ERR_POOL, "factor has more than 2^31 - 2 off-diagonal entries (int32 CSR)");
             h->fstats.nnz_factor = (int64_t)c[kCtrNnzOff] + h->nf;
             h->fstats.fills = (int64_t)c[kCtrFills];
             h->fstats.max_degree = (int64_t)c[kCtrMaxDeg];
@@ -864,7 +727,11 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         }
         if (st & kStatusInconsistent)
             return fail(h, RC_ERR_STATE, "elimination kernel found an inconsistent fill list");
-        if (st & kStatusFillPool) h->pool_cap = 2 * (h->pool_cap - slack) + slack;
+        if (st & kStatusFillPool) {
+            if (h->pool_cap >= kMaxPoolRecords)
+                return fail(h, RC_ERR_POOL, "fill pool would exceed 2^31 - 1 records (int32 record indices)");
+            h->pool_cap = std::min<long long>(2 * (h->pool_cap - slack) + slack, kMaxPoolRecords);
+        }
         if (st & (kStatusOutPool | kStatusFillPool)) h->out_cap = h->pool_cap + h->nnz_low + slack;
         if (st & kStatusScratch) h->scratch_cap = (int)std::min<int64_t>(pow2_at_least((int64_t)c[kCtrScratchNeed]), 1 << 28);
         ++*regrowths;
@@ -1229,7 +1096,7 @@ static double factor_bytes(rc_handle_t h)
            s * (double)h->nf;
 }
 
-int rc_factor(rc_handle_t h, rc_factor_stats *st)
+static int rc_factor_impl(rc_handle_t h, rc_factor_stats *st)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -1240,6 +1107,7 @@ int rc_factor(rc_handle_t h, rc_factor_stats *st)
     double fms = 0.0, ams = 0.0;
     int regrow = 0;
     h->factored = false;
+    h->m_solved = false;
     invalidate_graph(h);
     rc = run_elimination(h, &fms, &regrow);
     if (rc != RC_OK) return rc;
@@ -1282,7 +1150,7 @@ static int fetch_raw_factor(rc_handle_t h, std::vector<long long> &start, std::v
     return RC_OK;
 }
 
-int rc_get_factor(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *rowptr, int *colind, double *val, double *diag)
+static int rc_get_factor_impl(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *rowptr, int *colind, double *val, double *diag)
 {
     RC_CHECK_HANDLE();
     if (!h->factored) return fail(h, RC_ERR_STATE, "no factor");
@@ -1323,7 +1191,7 @@ int rc_get_factor(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *rowptr, int 
     return RC_OK;
 }
 
-int rc_dump_factor(rc_handle_t h, const char *prefix)
+static int rc_dump_factor_impl(rc_handle_t h, const char *prefix)
 {
     RC_CHECK_HANDLE();
     if (!prefix) return fail(h, RC_ERR_ARG, "null prefix");
@@ -1359,7 +1227,7 @@ int rc_dump_factor(rc_handle_t h, const char *prefix)
     return RC_OK;
 }
 
-int rc_load_factor(rc_handle_t h, const char *prefix)
+static int rc_load_factor_impl(rc_handle_t h, const char *prefix)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -1369,7 +1237,7 @@ int rc_load_factor(rc_handle_t h, const char *prefix)
     if (rc != RC_OK) return rc;
     std::string p(prefix);
     MtxTriplets t;
-    rc = read_mtx(p + "l_sol.mtx", t, h->err);
+    rc = read_mtx(p + "l_sol.mtx", t, h->err, h->opt.host_threads);
     if (rc != RC_OK) return rc;
     const int nf = h->nf;
     if (t.nrows != nf || t.ncols != nf) return fail(h, RC_ERR_FORMAT, "factor dimension does not match the system");
@@ -1390,6 +1258,7 @@ int rc_load_factor(rc_handle_t h, const char *prefix)
     std::vector<long long> start((size_t)nf, 0);
     long long tot = 0;
     for (int k = 0; k < nf; ++k) { start[(size_t)k] = tot; tot += cnt[(size_t)k]; }
+    if (tot > kMaxFactorEntries) return fail(h, RC_ERR_FORMAT, "factor has more than 2^31 - 2 off-diagonal entries");
     std::vector<int> row((size_t)tot);
     std::vector<double> val((size_t)tot);
     std::vector<long long> pos(start);
@@ -1579,7 +1448,7 @@ static int alloc_solve(rc_handle_t h)
     return RC_OK;
 }
 
-int rc_make_rhs(rc_handle_t h, int kind, uint64_t seed, double *b)
+static int rc_make_rhs_impl(rc_handle_t h, int kind, uint64_t seed, double *b)
 {
     RC_CHECK_HANDLE();
     if (h->kind < 0 || !b) return fail(h, RC_ERR_ARG, "no matrix or null array");
@@ -1612,7 +1481,7 @@ int rc_make_rhs(rc_handle_t h, int kind, uint64_t seed, double *b)
     return RC_OK;
 }
 
-int rc_set_reference_solution(rc_handle_t h, const double *x_ref)
+static int rc_set_reference_solution_impl(rc_handle_t h, const double *x_ref)
 {
     RC_CHECK_HANDLE();
     if (!x_ref) { h->xref.clear(); return RC_OK; }
@@ -1819,7 +1688,7 @@ static double a_norm_err(rc_handle_t h, const double *x)
     return den > 0.0 ? std::sqrt(std::max(num, 0.0) / den) : -1.0;
 }
 
-int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *st)
+static int rc_solve_impl(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *st)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -1838,7 +1707,7 @@ int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_sta
     return rc;
 }
 
-int rc_solve_batch(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *per_system)
+static int rc_solve_batch_impl(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *per_system)
 {
     RC_CHECK_HANDLE();
     if (h->nsys <= 0) return fail(h, RC_ERR_STATE, "handle holds no batch (rc_set_batch)");
@@ -1850,7 +1719,7 @@ int rc_solve_batch(rc_handle_t h, const double *b, double *x, double tol, rc_sol
     return rc;
 }
 
-int rc_get_batch_stats(rc_handle_t h, rc_solve_stats *per_system)
+static int rc_get_batch_stats_impl(rc_handle_t h, rc_solve_stats *per_system)
 {
     RC_CHECK_HANDLE();
     if (h->nsys <= 0 || h->bstats.size() != (size_t)h->nsys) return fail(h, RC_ERR_STATE, "no batched solve");
@@ -1958,7 +1827,7 @@ static int multi_args(rc_handle_t h, int k, MultiArgs &a)
     return RC_OK;
 }
 
-int rc_set_rhs_multi(rc_handle_t h, int k, const double *B)
+static int rc_set_rhs_multi_impl(rc_handle_t h, int k, const double *B)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -1971,17 +1840,20 @@ int rc_set_rhs_multi(rc_handle_t h, int k, const double *B)
     RC_HIP(hipMemcpyAsync(h->m_B.p, B, n * (size_t)k * 8, hipMemcpyHostToDevice, h->stream));
     RC_HIP(hipStreamSynchronize(h->stream));
     h->m_k = k;
+    h->m_n = (int)n;
+    h->m_solved = false;
     return RC_OK;
 }
 
-int rc_solve_multi_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+static int rc_solve_multi_resident_impl(rc_handle_t h, double tol, rc_solve_stats *st)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
     if (!(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad tolerance");
     if (!h->factored || h->m_k <= 0) return fail(h, RC_ERR_STATE, "rc_factor and rc_set_rhs_multi must precede");
     if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
-    if ((size_t)h->ns != (size_t)h->n_user) return fail(h, RC_ERR_STATE, "right-hand sides do not match the system");
+    if ((size_t)h->ns != (size_t)h->n_user || h->m_n != h->n_user)
+        return fail(h, RC_ERR_STATE, "right-hand sides do not match the system");
     RC_HIP(hipSetDevice(h->opt.device));
     hipStream_t s = h->stream;
     hipEvent_t e0, e1;
@@ -2034,8 +1906,9 @@ int rc_solve_multi_resident(rc_handle_t h, double tol, rc_solve_stats *st)
         any_break |= sc.breakdown[q];
         if (!st) continue;
         st[q].iterations = sc.iter[q];
-        bool met = h->opt.stop_rule == RC_STOP_REFERENCE ? (sc.rz0[q] <= tol || std::sqrt(sc.rz[q]) / std::sqrt(sc.rz0[q]) <= tol)
-                                                          : true;
+        bool met = h->opt.stop_rule == RC_STOP_REFERENCE
+                       ? (sc.rz0[q] <= tol || std::sqrt(sc.rz[q]) / std::sqrt(sc.rz0[q]) <= tol)
+                       : (sc.bb[q] <= 0.0 || std::sqrt(sc.rr[q]) / std::sqrt(sc.bb[q]) <= tol);
         st[q].converged = (sc.done[q] && !sc.breakdown[q] && (sc.iter[q] < h->opt.maxit || met)) ? 1 : 0;
         st[q].rel_res_true = sc.bb[q] > 0.0 ? std::sqrt(sc.res2[q]) / std::sqrt(sc.bb[q]) : 0.0;
         st[q].rel_res_precond = sc.rz0[q] > 0.0 ? std::sqrt(std::max(sc.rz[q], 0.0)) / std::sqrt(sc.rz0[q]) : 0.0;
@@ -2048,11 +1921,11 @@ int rc_solve_multi_resident(rc_handle_t h, double tol, rc_solve_stats *st)
     return RC_OK;
 }
 
-int rc_get_solution_multi(rc_handle_t h, double *X)
+static int rc_get_solution_multi_impl(rc_handle_t h, double *X)
 {
     RC_CHECK_HANDLE();
     if (!X) return fail(h, RC_ERR_ARG, "null X");
-    if (!h->m_solved || h->m_k <= 0) return fail(h, RC_ERR_STATE, "no multi-RHS solve");
+    if (!h->m_solved || h->m_k <= 0 || h->m_n != h->ns) return fail(h, RC_ERR_STATE, "no multi-RHS solve");
     RC_HIP(hipSetDevice(h->opt.device));
     const int k = h->m_k;
     int K = 1;
@@ -2065,7 +1938,7 @@ int rc_get_solution_multi(rc_handle_t h, double *X)
     return RC_OK;
 }
 
-int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
+static int rc_solve_multi_impl(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
 {
     RC_CHECK_HANDLE();
     if (!B || !X) return fail(h, RC_ERR_ARG, "bad multi-RHS arguments");
@@ -2081,7 +1954,7 @@ int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol,
 // ------------------------------------------------------------------ timing
 static const size_t kCalibBytes = (size_t)1 << 30;   // copy kernel: 1 GiB read + 1 GiB written
 
-int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
+static int rc_time_kernel_impl(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -2174,7 +2047,7 @@ int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, d
 
 // Resident-input variants used by the benchmark: the RHS is uploaded and
 // permuted before the timed region, the solution stays on the device.
-int rc_set_rhs(rc_handle_t h, const double *b)
+static int rc_set_rhs_impl(rc_handle_t h, const double *b)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -2187,7 +2060,7 @@ int rc_set_rhs(rc_handle_t h, const double *b)
     return RC_OK;
 }
 
-int rc_solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+static int rc_solve_resident_impl(rc_handle_t h, double tol, rc_solve_stats *st)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -2200,7 +2073,7 @@ int rc_solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
 // hipEvents, `reps` launches each) for every candidate (tail width limit x
 // sweep grid) and keep the fastest.  Changing the tail limit re-runs only the
 // assembly (the elimination's output stays on the device).
-int rc_autotune(rc_handle_t h, int reps, rc_tune_stats *st)
+static int rc_autotune_impl(rc_handle_t h, int reps, rc_tune_stats *st)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
@@ -2256,12 +2129,210 @@ int rc_autotune(rc_handle_t h, int reps, rc_tune_stats *st)
     return RC_OK;
 }
 
-int rc_get_solution(rc_handle_t h, double *x)
+static int rc_get_solution_impl(rc_handle_t h, double *x)
 {
     RC_CHECK_HANDLE();
     if (!x) return fail(h, RC_ERR_ARG, "null x");
     if (!h->rhs_set) return fail(h, RC_ERR_STATE, "no solve");
     return fetch_solution(h, x);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ABI guard
+// No C++ exception crosses the C ABI: host allocation failures become
+// RC_ERR_NOMEM, any other exception RC_ERR_ARG, with the text in rc_last_error.
+template <typename F>
+static int guarded(rc_handle_t h, F &&f) noexcept
+{
+    const char *what = "unknown exception";
+    int code = RC_ERR_ARG;
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        what = "host memory allocation failed";
+        code = RC_ERR_NOMEM;
+    } catch (const std::length_error &) {
+        what = "host container size limit exceeded";
+        code = RC_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        what = e.what();
+    } catch (...) {
+    }
+    if (h) {
+        try {
+            h->err = what;
+        } catch (...) {
+        }
+    }
+    return code;
+}
+
+extern "C" {
+
+int rc_create(rc_handle_t *out, const rc_options *opt)
+{
+    return guarded(nullptr, [&]() { return rc_create_impl(out, opt); });
+}
+
+int rc_destroy(rc_handle_t h)
+{
+    return guarded(h, [&]() { return rc_destroy_impl(h); });
+}
+
+int rc_set_triplets(rc_handle_t h, int n, int64_t m, const int *rows, const int *cols, const double *vals, int kind)
+{
+    return guarded(h, [&]() { return rc_set_triplets_impl(h, n, m, rows, cols, vals, kind); });
+}
+
+int rc_set_batch(rc_handle_t h, int nsys, const int *sys_n, const int64_t *sys_m, const int *rows, const int *cols, const double *vals)
+{
+    return guarded(h, [&]() { return rc_set_batch_impl(h, nsys, sys_n, sys_m, rows, cols, vals); });
+}
+
+int rc_get_batch(rc_handle_t h, int *nsys, int64_t *offsets)
+{
+    return guarded(h, [&]() { return rc_get_batch_impl(h, nsys, offsets); });
+}
+
+int rc_load_mtx(rc_handle_t h, const char *path, int kind)
+{
+    return guarded(h, [&]() { return rc_load_mtx_impl(h, path, kind); });
+}
+
+int rc_get_size(rc_handle_t h, int64_t *n, int64_t *nnz)
+{
+    return guarded(h, [&]() { return rc_get_size_impl(h, n, nnz); });
+}
+
+int rc_order(rc_handle_t h, int method, uint64_t seed)
+{
+    return guarded(h, [&]() { return rc_order_impl(h, method, seed); });
+}
+
+int rc_get_permutation(rc_handle_t h, int *perm)
+{
+    return guarded(h, [&]() { return rc_get_permutation_impl(h, perm); });
+}
+
+int rc_set_gnn_weights(rc_handle_t h, const float *w, int64_t count)
+{
+    return guarded(h, [&]() { return rc_set_gnn_weights_impl(h, w, count); });
+}
+
+int rc_set_permutation(rc_handle_t h, const int *perm)
+{
+    return guarded(h, [&]() { return rc_set_permutation_impl(h, perm); });
+}
+
+int rc_gnn_scores(rc_handle_t h, const float *w, float *scores)
+{
+    return guarded(h, [&]() { return rc_gnn_scores_impl(h, w, scores); });
+}
+
+int rc_gnn_gradient(rc_handle_t h, const float *w, const float *dscores, float *dw)
+{
+    return guarded(h, [&]() { return rc_gnn_gradient_impl(h, w, dscores, dw); });
+}
+
+int rc_get_prepared_csc(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *colptr, int *rowind, double *val)
+{
+    return guarded(h, [&]() { return rc_get_prepared_csc_impl(h, n, nnz, colptr, rowind, val); });
+}
+
+int rc_write_prepared_mtx(rc_handle_t h, const char *path)
+{
+    return guarded(h, [&]() { return rc_write_prepared_mtx_impl(h, path); });
+}
+
+int rc_factor(rc_handle_t h, rc_factor_stats *st)
+{
+    return guarded(h, [&]() { return rc_factor_impl(h, st); });
+}
+
+int rc_get_factor(rc_handle_t h, int64_t *n, int64_t *nnz, int64_t *rowptr, int *colind, double *val, double *diag)
+{
+    return guarded(h, [&]() { return rc_get_factor_impl(h, n, nnz, rowptr, colind, val, diag); });
+}
+
+int rc_dump_factor(rc_handle_t h, const char *prefix)
+{
+    return guarded(h, [&]() { return rc_dump_factor_impl(h, prefix); });
+}
+
+int rc_load_factor(rc_handle_t h, const char *prefix)
+{
+    return guarded(h, [&]() { return rc_load_factor_impl(h, prefix); });
+}
+
+int rc_make_rhs(rc_handle_t h, int kind, uint64_t seed, double *b)
+{
+    return guarded(h, [&]() { return rc_make_rhs_impl(h, kind, seed, b); });
+}
+
+int rc_set_reference_solution(rc_handle_t h, const double *x_ref)
+{
+    return guarded(h, [&]() { return rc_set_reference_solution_impl(h, x_ref); });
+}
+
+int rc_solve(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *st)
+{
+    return guarded(h, [&]() { return rc_solve_impl(h, b, x, tol, st); });
+}
+
+int rc_solve_batch(rc_handle_t h, const double *b, double *x, double tol, rc_solve_stats *per_system)
+{
+    return guarded(h, [&]() { return rc_solve_batch_impl(h, b, x, tol, per_system); });
+}
+
+int rc_get_batch_stats(rc_handle_t h, rc_solve_stats *per_system)
+{
+    return guarded(h, [&]() { return rc_get_batch_stats_impl(h, per_system); });
+}
+
+int rc_set_rhs_multi(rc_handle_t h, int k, const double *B)
+{
+    return guarded(h, [&]() { return rc_set_rhs_multi_impl(h, k, B); });
+}
+
+int rc_solve_multi_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    return guarded(h, [&]() { return rc_solve_multi_resident_impl(h, tol, st); });
+}
+
+int rc_get_solution_multi(rc_handle_t h, double *X)
+{
+    return guarded(h, [&]() { return rc_get_solution_multi_impl(h, X); });
+}
+
+int rc_solve_multi(rc_handle_t h, int k, const double *B, double *X, double tol, rc_solve_stats *st)
+{
+    return guarded(h, [&]() { return rc_solve_multi_impl(h, k, B, X, tol, st); });
+}
+
+int rc_time_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
+{
+    return guarded(h, [&]() { return rc_time_kernel_impl(h, kernel, reps, ms_per_launch, bytes_per_launch); });
+}
+
+int rc_set_rhs(rc_handle_t h, const double *b)
+{
+    return guarded(h, [&]() { return rc_set_rhs_impl(h, b); });
+}
+
+int rc_solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    return guarded(h, [&]() { return rc_solve_resident_impl(h, tol, st); });
+}
+
+int rc_autotune(rc_handle_t h, int reps, rc_tune_stats *st)
+{
+    return guarded(h, [&]() { return rc_autotune_impl(h, reps, st); });
+}
+
+int rc_get_solution(rc_handle_t h, double *x)
+{
+    return guarded(h, [&]() { return rc_get_solution_impl(h, x); });
 }
 
 }  // extern "C"

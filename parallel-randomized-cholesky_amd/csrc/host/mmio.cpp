@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <memory>
+#include <thread>
+#include <vector>
 
 #include "../../../include/rchol.h"
 
@@ -41,7 +44,7 @@ inline const char *skip_ws(const char *p)
 
 }  // namespace
 
-int read_mtx(const std::string &path, MtxTriplets &t, std::string &err)
+int read_mtx(const std::string &path, MtxTriplets &t, std::string &err, int threads)
 {
     std::unique_ptr<FILE, FileCloser> f(fopen(path.c_str(), "rb"));
     if (!f) { err = "cannot open " + path + ": " + strerror(errno); return RC_ERR_IO; }
@@ -94,30 +97,104 @@ int read_mtx(const std::string &path, MtxTriplets &t, std::string &err)
     t.nrows = (int)nr;
     t.ncols = (int)nc;
     t.rows.clear(); t.cols.clear(); t.vals.clear();
-    size_t cap = (size_t)nz * (sym ? 2 : 1);
+    // Entries are parsed in parallel: the entry text is cut into pieces at
+    // line ends, every thread parses its piece into its own arrays, and the
+    // pieces are concatenated in file order.  A piece's entries are counted
+    // as they are parsed, so the first nz entries of the file are kept.
+    const char *body = p;
+    const size_t blen = strlen(body);
+    int nthreads = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+    nthreads = std::max(1, std::min(nthreads, 64));
+    if (blen < ((size_t)1 << 20)) nthreads = 1;      // small files: one piece
+    std::vector<size_t> cut((size_t)nthreads + 1, blen);
+    cut[0] = 0;
+    for (int k = 1; k < nthreads; ++k) {
+        size_t c = std::max(cut[(size_t)k - 1], blen * (size_t)k / (size_t)nthreads);
+        while (c < blen && body[c - 1] != '\n') ++c;
+        cut[(size_t)k] = c;
+    }
+    struct Piece {
+        std::vector<int> r, c;
+        std::vector<double> v;
+        long long bad_line = -1;       // first malformed line of the piece (0-based within it)
+        int bad_kind = 0;              // 1 truncated, 2 missing value, 3 index out of range
+    };
+    std::vector<Piece> pieces((size_t)nthreads);
+    auto parse = [&](int k) {
+        Piece &pc = pieces[(size_t)k];
+        const char *q = body + cut[(size_t)k];
+        const char *qe = body + cut[(size_t)k + 1];
+        // an entry takes at least 4 bytes ("1 1\n"), 6 with a value
+        const size_t guess = (size_t)(qe - q) / (pattern ? 4 : 6) + 1;
+        pc.r.reserve(guess); pc.c.reserve(guess); pc.v.reserve(guess);
+        long long line = 0;
+        char *end = nullptr;
+        while (q < qe) {
+            const char *s0 = skip_ws(q);
+            if (s0 >= qe) break;
+            if (*s0 == '\n') { q = s0 + 1; ++line; continue; }
+            long long r = strtoll(s0, &end, 10);
+            if (end == s0) { pc.bad_line = line; pc.bad_kind = 1; return; }
+            const char *s1 = end;
+            long long c = strtoll(s1, &end, 10);
+            if (end == s1) { pc.bad_line = line; pc.bad_kind = 1; return; }
+            double v = 1.0;
+            if (!pattern) {
+                const char *s2 = end;
+                v = strtod(s2, &end);
+                if (end == s2) { pc.bad_line = line; pc.bad_kind = 2; return; }
+            }
+            if (end > qe) { pc.bad_line = line; pc.bad_kind = 1; return; }   // ran into the next piece
+            if (r < 1 || r > nr || c < 1 || c > nc) { pc.bad_line = line; pc.bad_kind = 3; return; }
+            pc.r.push_back((int)(r - 1));
+            pc.c.push_back((int)(c - 1));
+            pc.v.push_back(v);
+            const char *nl = (const char *)memchr(end, '\n', (size_t)(qe - end));
+            q = nl ? nl + 1 : qe;
+            ++line;
+        }
+    };
+    if (nthreads == 1) {
+        parse(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int k = 0; k < nthreads; ++k) th.emplace_back(parse, k);
+        for (auto &x : th) x.join();
+    }
+    long long got = 0;
+    for (int k = 0; k < nthreads; ++k) {
+        const Piece &pc = pieces[(size_t)k];
+        if (pc.bad_kind != 0 && got + (long long)pc.r.size() < nz) {
+            const long long e = got + (long long)pc.r.size();
+            if (pc.bad_kind == 1) err = "truncated entry list at entry " + std::to_string(e);
+            else if (pc.bad_kind == 2) err = "missing value at entry " + std::to_string(e);
+            else err = "index out of range at entry " + std::to_string(e);
+            return RC_ERR_FORMAT;
+        }
+        got += (long long)pc.r.size();
+        if (pc.bad_kind != 0 || got >= nz) break;
+    }
+    if (got < nz) { err = "truncated entry list at entry " + std::to_string(got); return RC_ERR_FORMAT; }
+    const size_t cap = (size_t)nz * (sym ? 2 : 1);
     t.rows.reserve(cap); t.cols.reserve(cap); t.vals.reserve(cap);
-    for (long long e = 0; e < nz; ++e) {
-        long long r = strtoll(p, &end, 10);
-        if (end == p) { err = "truncated entry list at entry " + std::to_string(e); return RC_ERR_FORMAT; }
-        p = end;
-        long long c = strtoll(p, &end, 10);
-        if (end == p) { err = "truncated entry list at entry " + std::to_string(e); return RC_ERR_FORMAT; }
-        p = end;
-        double v = 1.0;
-        if (!pattern) {
-            v = strtod(p, &end);
-            if (end == p) { err = "missing value at entry " + std::to_string(e); return RC_ERR_FORMAT; }
-            p = end;
+    long long left = nz;
+    for (int k = 0; k < nthreads && left > 0; ++k) {
+        Piece &pc = pieces[(size_t)k];
+        const size_t take = (size_t)std::min<long long>(left, (long long)pc.r.size());
+        for (size_t e = 0; e < take; ++e) {
+            t.rows.push_back(pc.r[e]);
+            t.cols.push_back(pc.c[e]);
+            t.vals.push_back(pc.v[e]);
+            if (sym && pc.r[e] != pc.c[e]) {
+                t.rows.push_back(pc.c[e]);
+                t.cols.push_back(pc.r[e]);
+                t.vals.push_back(sym == 2 ? -pc.v[e] : pc.v[e]);
+            }
         }
-        if (r < 1 || r > nr || c < 1 || c > nc) { err = "index out of range at entry " + std::to_string(e); return RC_ERR_FORMAT; }
-        t.rows.push_back((int)(r - 1));
-        t.cols.push_back((int)(c - 1));
-        t.vals.push_back(v);
-        if (sym && r != c) {
-            t.rows.push_back((int)(c - 1));
-            t.cols.push_back((int)(r - 1));
-            t.vals.push_back(sym == 2 ? -v : v);
-        }
+        left -= (long long)take;
+        std::vector<int>().swap(pc.r);
+        std::vector<int>().swap(pc.c);
+        std::vector<double>().swap(pc.v);
     }
     return RC_OK;
 }

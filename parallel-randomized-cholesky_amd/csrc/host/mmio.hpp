@@ -15,7 +15,8 @@ struct MtxTriplets {
     std::vector<double> vals;
 };
 
-int read_mtx(const std::string &path, MtxTriplets &t, std::string &err);
+// threads: host threads of the entry parser (0 = all hardware threads).
+int read_mtx(const std::string &path, MtxTriplets &t, std::string &err, int threads = 0);
 int write_mtx_csc(const std::string &path, int nrows, int ncols, const int64_t *colptr, const int *rowind,
                   const double *val, std::string &err);
 

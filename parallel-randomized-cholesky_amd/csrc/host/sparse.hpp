@@ -46,6 +46,9 @@ std::vector<int> nnz_sort_perm(const Csc &a, int nvert, uint64_t seed, int count
 // Approximate minimum degree (Amestoy, Davis, Duff 1996) on the pattern of
 // the first nvert vertices.
 std::vector<int> amd_perm(const Csc &a, int nvert);
+// Nested dissection by BFS level-structure separators (nd.cpp) on the first
+// nvert vertices (ties broken by `seed`), `threads` host threads (0 = all).
+std::vector<int> nd_perm(const Csc &a, int nvert, uint64_t seed, int threads);
 
 // Strict lower part as int32 CSC with |values| (cpu pre_process.hpp:763-850).
 int lower_part(const Csc &a, std::vector<int> &ptr, std::vector<int> &row, std::vector<double> &w);
