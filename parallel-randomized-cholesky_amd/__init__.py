@@ -40,7 +40,7 @@ EXPORTS = (
     "rc_get_solution", "rc_time_kernel", "rc_get_prepared_csc", "rc_write_prepared_mtx", "rc_solve_multi",
     "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
     "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats", "rc_autotune",
-    "rc_set_rhs_multi", "rc_solve_multi_resident", "rc_get_solution_multi",
+    "rc_set_rhs_multi", "rc_solve_multi_resident", "rc_get_solution_multi", "rc_solve_noalloc",
 )
 
 
@@ -150,6 +150,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_set_rhs_multi.argtypes = [vp, ctypes.c_int, vp]
     lib.rc_solve_multi_resident.argtypes = [vp, ctypes.c_double, vp]
     lib.rc_get_solution_multi.argtypes = [vp, vp]
+    lib.rc_solve_noalloc.argtypes = [vp, vp, vp, ctypes.c_double, P(SolveStats)]
     _lib = lib
     return lib
 
@@ -372,6 +373,13 @@ class Solver:
         Xc = np.empty((self._k, n), dtype=np.float64)
         self._check(self.lib.rc_get_solution_multi(self.h, _ptr(Xc)))
         return Xc.T.copy()
+
+    def solve_noalloc(self, b_dev, x_dev, tol=1e-6):
+        """PCG on device-resident b / x (device addresses as ints); allocates nothing."""
+        st = SolveStats()
+        self._check(self.lib.rc_solve_noalloc(self.h, ctypes.c_void_p(b_dev), ctypes.c_void_p(x_dev), float(tol),
+                                              ctypes.byref(st)))
+        return st.as_dict()
 
     def set_rhs(self, b):
         self._b = np.ascontiguousarray(b, dtype=np.float64)

@@ -425,6 +425,16 @@ void launch_batch_residual(const SolveArgs &a, const BatchArgs &b, hipStream_t s
 void launch_level_keys_sys(int ns, const int *level, const int *lptr, int len_bits, const int *sys, int sys_bits,
                            uint32_t *keys, hipStream_t s);
 
+// ------------------------------------------------------------ fixed workspace
+// Device workspace of rc_solve_noalloc: a file-scope __device__ array of
+// kNoAllocScratchBytes per GPU (noalloc.hip).  noalloc_workspace returns its
+// device address on `device`, which must be the current device; the address
+// is resolved once per device and cached.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t noalloc_workspace(int device, void **addr);
+// v[n] = 0 (one thread)
+void launch_zero_slot(double *v, int n, hipStream_t s);
+
 // CSR in position numbering: row p = source row order[p], columns mapped by
 // src.colmap, values converted to TD.
 void launch_pos_csr_count(int n, const int *order, const int *src_ptr, int *cnt, hipStream_t s);

@@ -96,6 +96,9 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, o.device) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
     h->spin_ticks = (long long)rate_khz * 1000LL * 20LL;        // 20 s
+    // device address of the fixed workspace of rc_solve_noalloc, resolved once
+    // per device with the device current (an error is reported by that entry point)
+    h->na_err = noalloc_workspace(o.device, &h->na_ws);
     h->num_blocks = o.num_blocks > 0 ? o.num_blocks : factor_occupancy_blocks(o.device);
     factor_kernel_setup();
     h->sweep_blocks = sweep_occupancy_blocks(o.device);
@@ -1325,7 +1328,9 @@ static int rc_load_factor_impl(rc_handle_t h, const char *prefix)
 }
 
 // ------------------------------------------------------------------ solve
-static SolveArgs solve_args(rc_handle_t h)
+}  // extern "C"
+
+SolveArgs solve_args(rc_handle_t h)
 {
     SolveArgs a;
     a.n = h->ns;
@@ -1401,6 +1406,8 @@ static SolveArgs solve_args(rc_handle_t h)
     }
     return a;
 }
+
+extern "C" {
 
 static BatchArgs batch_args(rc_handle_t h)
 {
@@ -2138,35 +2145,6 @@ static int rc_get_solution_impl(rc_handle_t h, double *x)
 }
 
 }  // extern "C"
-
-// ------------------------------------------------------------------ ABI guard
-// No C++ exception crosses the C ABI: host allocation failures become
-// RC_ERR_NOMEM, any other exception RC_ERR_ARG, with the text in rc_last_error.
-template <typename F>
-static int guarded(rc_handle_t h, F &&f) noexcept
-{
-    const char *what = "unknown exception";
-    int code = RC_ERR_ARG;
-    try {
-        return f();
-    } catch (const std::bad_alloc &) {
-        what = "host memory allocation failed";
-        code = RC_ERR_NOMEM;
-    } catch (const std::length_error &) {
-        what = "host container size limit exceeded";
-        code = RC_ERR_NOMEM;
-    } catch (const std::exception &e) {
-        what = e.what();
-    } catch (...) {
-    }
-    if (h) {
-        try {
-            h->err = what;
-        } catch (...) {
-        }
-    }
-    return code;
-}
 
 extern "C" {
 

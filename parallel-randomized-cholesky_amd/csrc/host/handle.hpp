@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -129,6 +131,10 @@ struct rc_handle_s {
     int gnn_n = 0;
     DevVec g_ptr, g_ind, g_a, g_at, g_H0, g_M1, g_H[3], g_M[3], g_Z[3], g_dA, g_dB, g_dC, g_part, g_w, g_dw, g_s, g_ds;
 
+    // fixed workspace of rc_solve_noalloc on this handle's device (resolved at rc_create)
+    void *na_ws = nullptr;
+    hipError_t na_err = hipSuccess;
+
     // batched systems (nsys == 0: one system)
     int nsys = 0;
     std::vector<int64_t> sys_off;   // nsys + 1, user numbering
@@ -177,3 +183,34 @@ inline void invalidate_graph(rc_handle_t h)
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
 }
+
+// No C++ exception crosses the C ABI: host allocation failures become
+// RC_ERR_NOMEM, any other exception RC_ERR_ARG, with the text in rc_last_error.
+template <typename F>
+int guarded(rc_handle_t h, F &&f) noexcept
+{
+    const char *what = "unknown exception";
+    int code = RC_ERR_ARG;
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        what = "host memory allocation failed";
+        code = RC_ERR_NOMEM;
+    } catch (const std::length_error &) {
+        what = "host container size limit exceeded";
+        code = RC_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        what = e.what();
+    } catch (...) {
+    }
+    if (h) {
+        try {
+            h->err = what;
+        } catch (...) {
+        }
+    }
+    return code;
+}
+
+// Shared between the host sources of the ABI (capi.cpp).
+SolveArgs solve_args(rc_handle_t h);
