@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liboracle.so")
 
 MUT_NONE, MUT_FILL_WEIGHT, MUT_SAMPLE_SHIFT, MUT_NO_MERGE, MUT_PIVOT, MUT_PRECOND_SIGN = 0, 1, 2, 3, 4, 5
+MUT_IC0_NO_PIVOT = 6
 STOP_REFERENCE, STOP_TRUE_RES = 0, 1
 
 _i64 = np.ctypeslib.ndpointer(np.int64, flags="C")
@@ -80,6 +81,17 @@ def lib():
     L.orc_pcg.argtypes = [ctypes.c_int, _i64, _i32, _f64, _i64, _i32, _f64, _f64, ctypes.c_int, _f64, _f64,
                           ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PcgStats)]
     L.orc_pcg.restype = ctypes.c_int
+    L.orc_ic0.argtypes = [ctypes.c_int, _i64, _i32, _f64, ctypes.c_int, ctypes.c_int64, _i64, _i32, _f64, _f64, _i32]
+    L.orc_ic0.restype = ctypes.c_int
+    L.orc_normal_vector.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_uint32, _f64]
+    L.orc_normal_vector.restype = None
+    L.orc_factor_omp.argtypes = [ctypes.c_int, _i64, _i32, _f64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64,
+                                 _i64, _i32, _f64, _f64, _i32, ctypes.POINTER(FactorStats)]
+    L.orc_factor_omp.restype = ctypes.c_int
+    L.orc_pcg_omp.argtypes = [ctypes.c_int, _i64, _i32, _f64, _i64, _i32, _f64, _f64, _i32, _f64, _f64,
+                              ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(PcgStats)]
+    L.orc_pcg_omp.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -217,6 +229,64 @@ def factor(a: Csc, seed=1, mutation=MUT_NONE, cap=None):
     q = int(ptr[-1])
     stats = {k: getattr(st, k) for k, _ in st._fields_}
     return Factor(n, ptr, ind[:q].copy(), val[:q].copy(), diag, level, stats)
+
+
+def normal_vector(seed, n, stream=0):
+    """[normal(seed, i, stream) for i < n], computed in C."""
+    out = np.empty(int(n), np.float64)
+    lib().orc_normal_vector(seed, int(n), stream, out)
+    return out
+
+
+def factor_omp(a: Csc, seed=1, threads=0, cap=None):
+    """orc_factor scheduled on `threads` OpenMP threads (0 = all); bitwise equal to factor()."""
+    n = a.n
+    if cap is None:
+        cap = max(16 * a.nnz, 1024)
+    ptr = np.zeros(n + 1, np.int64)
+    ind = np.zeros(cap, np.int32)
+    val = np.zeros(cap, np.float64)
+    diag = np.zeros(n, np.float64)
+    level = np.zeros(n, np.int32)
+    st = FactorStats()
+    rc = lib().orc_factor_omp(n, a.ptr, a.ind, a.val, seed, int(threads), cap, ptr, ind, val, diag, level,
+                              ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("oracle factor capacity exceeded")
+    q = int(ptr[-1])
+    stats = {k: getattr(st, k) for k, _ in st._fields_}
+    return Factor(n, ptr, ind[:q], val[:q], diag, level, stats)
+
+
+def pcg_omp(a: Csc, f: Factor, b, tol=1e-6, maxit=300, threads=0):
+    """The reference PCG (FP64, reference stop rule) with OpenMP kernels and
+    level-scheduled triangular solves."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(a.n)
+    st = PcgStats()
+    setup = ctypes.c_double()
+    level = np.ascontiguousarray(f.level[:a.n], dtype=np.int32)
+    rc = lib().orc_pcg_omp(a.n, a.ptr, a.ind, a.val, f.ptr, f.ind, f.val, f.diag, level, b, x, tol, maxit,
+                           int(threads), ctypes.byref(setup), ctypes.byref(st))
+    out = {k: getattr(st, k) for k, _ in st._fields_}
+    out["status"] = rc
+    out["setup_s"] = setup.value
+    return x, out
+
+
+def ic0(a: Csc, mutation=MUT_NONE):
+    """IC(0) L D L^T of a (pattern of its lower triangle), orc_factor's layout."""
+    n = a.n
+    cap = max(a.nnz, 1)
+    ptr = np.zeros(n + 1, np.int64)
+    ind = np.zeros(cap, np.int32)
+    val = np.zeros(cap, np.float64)
+    diag = np.zeros(n, np.float64)
+    level = np.zeros(n, np.int32)
+    if lib().orc_ic0(n, a.ptr, a.ind, a.val, mutation, cap, ptr, ind, val, diag, level) != 0:
+        raise RuntimeError("ic0 capacity exceeded")
+    q = int(ptr[-1])
+    return Factor(n, ptr, ind[:q].copy(), val[:q].copy(), diag, level, {})
 
 
 def factor_trim_last(f: Factor):

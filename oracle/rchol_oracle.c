@@ -74,6 +74,11 @@ double orc_normal(uint64_t seed, uint32_t index, uint32_t stream)
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+void orc_normal_vector(uint64_t seed, int64_t n, uint32_t stream, double *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_normal(seed, (uint32_t)i, stream);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Small sort helpers (insertion sort for short runs, qsort beyond).        */
 /* ------------------------------------------------------------------------ */
@@ -556,5 +561,80 @@ int orc_pcg(int n, const int64_t *a_ptr, const int *a_ind, const double *a_val,
         st->rel_res_true = bnorm > 0.0 ? sqrt(dot(n, ap, ap)) / bnorm : 0.0;
     }
     free(r); free(z); free(p); free(ap);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IC(0): the context preconditioner (see rchol_oracle.h)                    */
+/* ------------------------------------------------------------------------ */
+int orc_ic0(int n, const int64_t *colptr, const int *rowind, const double *val, int mutation, int64_t cap,
+            int64_t *lt_ptr, int *lt_ind, double *lt_val, double *diag, int *level)
+{
+    /* row i of L = rows j < i of column i (the pattern is symmetric) */
+    int64_t *up = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int i = 0; i < n; ++i) {
+        int64_t c = 0;
+        for (int64_t p = colptr[i]; p < colptr[i + 1]; ++p) c += rowind[p] < i;
+        up[i + 1] = up[i] + c;
+    }
+    int *ui = (int *)malloc((size_t)(up[n] > 0 ? up[n] : 1) * sizeof(int));
+    double *uv = (double *)malloc((size_t)(up[n] > 0 ? up[n] : 1) * sizeof(double));
+    double *l = (double *)malloc((size_t)(up[n] > 0 ? up[n] : 1) * sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        int64_t q = up[i];
+        for (int64_t p = colptr[i]; p < colptr[i + 1]; ++p)
+            if (rowind[p] < i) { ui[q] = rowind[p]; uv[q] = val[p]; ++q; }
+    }
+    for (int i = 0; i < n; ++i) {
+        double aii = 0.0;
+        for (int64_t p = colptr[i]; p < colptr[i + 1]; ++p)
+            if (rowind[p] == i) aii += val[p];
+        double di = aii;
+        int lev = 0;
+        for (int64_t t = up[i]; t < up[i + 1]; ++t) {
+            const int j = ui[t];
+            double s = uv[t];
+            int64_t p = up[i], q = up[j];
+            while (p < t && q < up[j + 1]) {
+                if (ui[p] == ui[q]) {
+                    s -= mutation == ORC_MUT_IC0_NO_PIVOT ? l[p] * l[q] : l[p] * l[q] * diag[ui[p]];
+                    ++p;
+                    ++q;
+                }
+                else if (ui[p] < ui[q]) ++p;
+                else ++q;
+            }
+            const double dj = diag[j];
+            const double lij = dj > 0.0 ? s / dj : 0.0;
+            l[t] = lij;
+            di -= lij * lij * dj;
+            if (level[j] + 1 > lev) lev = level[j] + 1;
+        }
+        if (!(di > 1e-14 * aii)) di = 0.0;
+        diag[i] = di;
+        level[i] = lev;
+    }
+    /* columns of L (rows > k, ascending) */
+    lt_ptr[0] = 0;
+    for (int k = 0; k < n; ++k) {
+        int64_t c = 0;
+        for (int64_t p = colptr[k]; p < colptr[k + 1]; ++p) c += rowind[p] > k;
+        lt_ptr[k + 1] = lt_ptr[k] + c;
+    }
+    int rc = 0;
+    if (lt_ptr[n] > cap) rc = -1;
+    if (rc == 0) {
+        int64_t *pos = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+        memcpy(pos, lt_ptr, (size_t)n * sizeof(int64_t));
+        /* rows i ascending, so every column receives its rows in order */
+        for (int i = 0; i < n; ++i)
+            for (int64_t t = up[i]; t < up[i + 1]; ++t) {
+                const int64_t q = pos[ui[t]]++;
+                lt_ind[q] = i;
+                lt_val[q] = l[t];
+            }
+        free(pos);
+    }
+    free(up); free(ui); free(uv); free(l);
     return rc;
 }

@@ -49,6 +49,9 @@ enum { ORC_TAG_SAMPLE = 0, ORC_TAG_NORMAL = 1, ORC_TAG_PERM = 2, ORC_TAG_WEIGHT 
 /* Standard normal by Box-Muller (cosine branch) from one Philox block. */
 double orc_normal(uint64_t seed, uint32_t index, uint32_t stream);
 
+/* out[i] = orc_normal(seed, i, stream) for i < n. */
+void orc_normal_vector(uint64_t seed, int64_t n, uint32_t stream, double *out);
+
 /* Laplacian construction from MM triplets (pre_process.hpp:189-306, 322-357):
  * off-diagonals become -|v|, diagonals +|v|; if the input holds no diagonal
  * entry at all, a diagonal equal to the column sum of |off-diagonals| is
@@ -93,7 +96,8 @@ enum {
     ORC_MUT_SAMPLE_SHIFT = 2,  /* sampled index j+1 (clamped) instead of j       */
     ORC_MUT_NO_MERGE = 3,      /* keep only the first entry of each duplicate row */
     ORC_MUT_PIVOT = 4,         /* D_k = S without the last entry                 */
-    ORC_MUT_PRECOND_SIGN = 5   /* backward solve adds instead of subtracting     */
+    ORC_MUT_PRECOND_SIGN = 5,  /* backward solve adds instead of subtracting     */
+    ORC_MUT_IC0_NO_PIVOT = 6   /* IC(0): d_m left out of the update sums          */
 };
 
 typedef struct {
@@ -153,6 +157,35 @@ int orc_pcg(int n, const int64_t *a_ptr, const int *a_ind, const double *a_val,
             const int64_t *lt_ptr, const int *lt_ind, const double *lt_val, const double *diag,
             int precision, const double *b, double *x, double tol, int maxit, int stop_rule,
             orc_pcg_stats *st);
+
+/* Incomplete Cholesky IC(0) L D L^T on the pattern of the lower triangle
+ * (the context preconditioner; textbook recurrence, rows in index order):
+ *   l_ij = (a_ij - sum_{m<j, (i,m),(j,m) in pattern} l_im l_jm d_m) / d_j,
+ *   d_i  = a_ii - sum_j l_ij^2 d_j,
+ * l_ij = 0 where d_j <= 0, d_i = 0 where it is not above 1e-14 a_ii.
+ * Output in orc_factor's layout (off-diagonal L^T in CSR: row k lists the
+ * rows r > k of column k of L with l_rk; pivots; forward levels).  Capacity
+ * cap >= strict-lower nnz.  Returns 0, or -1 on capacity. */
+int orc_ic0(int n, const int64_t *colptr, const int *rowind, const double *val, int mutation, int64_t cap,
+            int64_t *lt_ptr, int *lt_ind, double *lt_val, double *diag, int *level);
+
+/* ---- multi-threaded restatement (rchol_omp.c; CPU baseline) ---------------
+ * orc_factor_omp: the same elimination as orc_factor, scheduled like the
+ * reference's OpenMP factorization (driver_local.cpp:302-707, 802-829):
+ * columns are taken from a shared ready queue as their lower neighbours
+ * retire.  Bitwise equal to orc_factor for every thread count (threads <= 0:
+ * all).  Returns 0, or -1 if the factor needs more than `cap` entries. */
+int orc_factor_omp(int n, const int64_t *colptr, const int *rowind, const double *val, uint64_t seed,
+                   int threads, int64_t cap, int64_t *lt_ptr, int *lt_ind, double *lt_val, double *diag,
+                   int *level, orc_factor_stats *st);
+
+/* orc_pcg_omp: the reference recurrence and stop rule (solver.hpp:457-569,
+ * FP64) with OpenMP SpMV / dots / updates and level-scheduled triangular
+ * solves over the factor's forward levels.  *setup_s = time to build the
+ * level lists and rows of L. */
+int orc_pcg_omp(int n, const int64_t *a_ptr, const int *a_ind, const double *a_val, const int64_t *lt_ptr,
+                const int *lt_ind, const double *lt_val, const double *diag, const int *level, const double *b,
+                double *x, double tol, int maxit, int threads, double *setup_s, orc_pcg_stats *st);
 
 #ifdef __cplusplus
 }

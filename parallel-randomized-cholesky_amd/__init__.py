@@ -30,6 +30,7 @@ ORDER_AS_IS, ORDER_NNZ_SORT, ORDER_RANDOM, ORDER_AMD, ORDER_LEARNED, ORDER_ND = 
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 RHS_ZERO_SUM_UNIFORM, RHS_NORMAL, RHS_NORMAL_PROJECTED = 0, 1, 2
 KERNEL_SPMV, KERNEL_TRSV_FWD, KERNEL_TRSV_BWD, KERNEL_PCG_ITER, KERNEL_COPY = 0, 1, 2, 3, 4
+KERNEL_MULTI_SPMM, KERNEL_MULTI_TRSV_FWD, KERNEL_MULTI_TRSV_BWD = 5, 6, 7
 
 # Every symbol include/rchol.h declares.
 EXPORTS = (
@@ -41,6 +42,7 @@ EXPORTS = (
     "rc_set_permutation", "rc_gnn_weight_count", "rc_gnn_scores", "rc_gnn_gradient",
     "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats", "rc_autotune",
     "rc_set_rhs_multi", "rc_solve_multi_resident", "rc_get_solution_multi", "rc_solve_noalloc",
+    "rc_get_multi_stats", "rc_solve_group", "rc_set_solve_options", "rc_factor_ic0",
 )
 
 
@@ -78,7 +80,7 @@ class FactorStats(ctypes.Structure):
         ("n", ctypes.c_int64), ("nnz_lower", ctypes.c_int64), ("nnz_factor", ctypes.c_int64),
         ("fills", ctypes.c_int64), ("max_degree", ctypes.c_int64), ("nlevels", ctypes.c_int32),
         ("zero_pivots", ctypes.c_int32), ("factor_ms", ctypes.c_double), ("assemble_ms", ctypes.c_double),
-        ("bytes_factor", ctypes.c_double), ("pool_regrowths", ctypes.c_int32),
+        ("bytes_factor", ctypes.c_double), ("pool_regrowths", ctypes.c_int32), ("prepare_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -151,6 +153,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_solve_multi_resident.argtypes = [vp, ctypes.c_double, vp]
     lib.rc_get_solution_multi.argtypes = [vp, vp]
     lib.rc_solve_noalloc.argtypes = [vp, vp, vp, ctypes.c_double, P(SolveStats)]
+    lib.rc_get_multi_stats.argtypes = [vp, vp]
+    lib.rc_solve_group.argtypes = [ctypes.c_int, vp, ctypes.c_double, vp]
+    lib.rc_set_solve_options.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.rc_factor_ic0.argtypes = [vp, P(FactorStats)]
     _lib = lib
     return lib
 
@@ -301,6 +307,12 @@ class Solver:
         self._check(self.lib.rc_factor(self.h, ctypes.byref(st)))
         return st.as_dict()
 
+    def factor_ic0(self):
+        """Incomplete Cholesky IC(0) of the prepared matrix as the preconditioner."""
+        st = FactorStats()
+        self._check(self.lib.rc_factor_ic0(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
     def get_factor(self):
         """CSR of L^T with unit diagonal first (reference layout) and the pivots."""
         n, nnz = ctypes.c_int64(), ctypes.c_int64()
@@ -368,6 +380,11 @@ class Solver:
             self._check(rc)
         return [st[q].as_dict() for q in range(self._k)]
 
+    def multi_stats(self):
+        st = (SolveStats * self._k)()
+        self._check(self.lib.rc_get_multi_stats(self.h, st))
+        return [st[q].as_dict() for q in range(self._k)]
+
     def get_solution_multi(self):
         n, _ = self.size()
         Xc = np.empty((self._k, n), dtype=np.float64)
@@ -380,6 +397,10 @@ class Solver:
         self._check(self.lib.rc_solve_noalloc(self.h, ctypes.c_void_p(b_dev), ctypes.c_void_p(x_dev), float(tol),
                                               ctypes.byref(st)))
         return st.as_dict()
+
+    def set_solve_options(self, stop_rule=STOP_REFERENCE, maxit=0):
+        """Stop rule and (maxit > 0) iteration limit of later solves."""
+        self._check(self.lib.rc_set_solve_options(self.h, stop_rule, maxit))
 
     def set_rhs(self, b):
         self._b = np.ascontiguousarray(b, dtype=np.float64)
@@ -406,3 +427,15 @@ class Solver:
         ms, by = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.rc_time_kernel(self.h, kernel, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+
+def solve_group(solvers, tol=1e-6, allow_breakdown=False):
+    """rc_solve_group over Solver objects whose right-hand sides are resident
+    (set_rhs or set_rhs_multi); returns one summary stats dict per solver."""
+    lib = load_library()
+    hs = (ctypes.c_void_p * len(solvers))(*[s.h.value for s in solvers])
+    st = (SolveStats * len(solvers))()
+    rc = lib.rc_solve_group(len(solvers), hs, float(tol), st)
+    if not (rc == OK or (allow_breakdown and rc == ERR_BREAKDOWN)):
+        raise RcholError(rc, lib.rc_last_error(solvers[0].h).decode(errors="replace"))
+    return [st[i].as_dict() for i in range(len(solvers))]

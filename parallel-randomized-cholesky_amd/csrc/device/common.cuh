@@ -62,6 +62,11 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
+__device__ __forceinline__ int wave_sum_int(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
 __device__ __forceinline__ int64_t wave_max(int64_t v)
 {
     for (int off = 32; off > 0; off >>= 1) {

@@ -392,6 +392,12 @@ struct MultiArgs {
 template <typename T> void launch_multi_init(const MultiArgs &a, hipStream_t s);
 template <typename T> void launch_multi_iteration(const MultiArgs &a, int parity, hipStream_t s);
 void launch_multi_residual(const MultiArgs &a, hipStream_t s);
+// Single kernels of a multi-RHS iteration, for timing: the SpMM Ap = A p0
+// (with its p'Ap partials), and the two sweeps (y, z sentinels first).
+template <typename T> void launch_multi_spmm_only(const MultiArgs &a, hipStream_t s);
+template <typename T> void launch_multi_fwd_only(const MultiArgs &a, hipStream_t s);
+template <typename T> void launch_multi_bwd_only(const MultiArgs &a, hipStream_t s);
+template <typename T> void launch_multi_sentinels(const MultiArgs &a, bool y, bool z, hipStream_t s);
 void launch_multi_gather(const double *B, const int *idx_user, int n, int k, int K, double *b, hipStream_t s);
 void launch_multi_scatter(const double *x, const int *idx_user, int n, int k, int K, double *X, hipStream_t s);
 
@@ -424,6 +430,42 @@ template <typename T> void launch_batch_iteration(const SolveArgs &a, const Batc
 void launch_batch_residual(const SolveArgs &a, const BatchArgs &b, hipStream_t s);
 void launch_level_keys_sys(int ns, const int *level, const int *lptr, int len_bits, const int *sys, int sys_bits,
                            uint32_t *keys, hipStream_t s);
+
+// ------------------------------------------------------------ input preparation (prep.hip)
+void launch_map_rows(int nnz, const int *ind, const int *inv, uint32_t *rowk, hipStream_t s);
+void launch_gather_key(const int *src, const int *idx, uint32_t *dst, int n, hipStream_t s);
+void launch_perm_finish(int nnz, const int *idx, const int *ind, const int *inv, const double *val, int *out_ind,
+                        double *out_val, hipStream_t s);
+void launch_colsum(int n, const int *ptr, const double *val, double *cs, double *app, int *flag, hipStream_t s);
+void launch_ground_count(int n, const int *ptr, const int *flag, int extra, int *cnt, hipStream_t s);
+void launch_ground_fill(int n, const int *ptr, const int *ind, const double *val, const double *app, const int *flag,
+                        const int *fpos, double corner, const int *optr, int *oind, double *oval, hipStream_t s);
+void launch_trim_count(int n, const int *ptr, const int *ind, int *cnt, hipStream_t s);
+void launch_trim_fill(int n, const int *ptr, const int *ind, const double *val, const int *optr, int *oind,
+                      double *oval, hipStream_t s);
+void launch_lower_count(int n, const int *ptr, const int *ind, int *cnt, int *dep, hipStream_t s);
+void launch_lower_fill(int n, const int *ptr, const int *ind, const double *val, const int *lo_ptr, int *lo_row,
+                       double *lo_w, hipStream_t s);
+void launch_gumbel_keys(int n, const float *score, uint64_t seed, uint32_t *hi, uint32_t *lo, hipStream_t s);
+
+// ------------------------------------------------------------ IC(0) (ic0.hip)
+struct Ic0Args {
+    int n;
+    const int *up_ptr, *up_ind;   // row i of L: the rows j < i of column i of the prepared matrix
+    const double *up_val, *diag_a;
+    double *l;                    // values of L in the up_* layout
+    double *d;                    // pivots, sentinel until published
+    int *level;                   // forward-solve level of every row
+    unsigned int *claim, *status; // chunk claim counter (zeroed), bit 0 = a wait expired
+    long long spin_ticks;
+};
+void launch_ic0_upper(int n, const int *ptr, const int *ind, const double *val, const int *up_ptr, int *up_ind,
+                      double *up_val, double *diag_a, hipStream_t s);
+void launch_ic0(const Ic0Args &a, int waves, hipStream_t s);
+void launch_ic0_scatter(int n, const int *up_ptr, const int *up_ind, const double *l, const int *lo_ptr,
+                        const int *lo_row, double *out_val, hipStream_t s);
+// out_start[k] = lo_ptr[k], out_cnt[k] = lo_ptr[k+1] - lo_ptr[k]
+void launch_ptr_to_start_cnt(int n, const int *lo_ptr, long long *out_start, int *out_cnt, hipStream_t s);
 
 // ------------------------------------------------------------ fixed workspace
 // Device workspace of rc_solve_noalloc: a file-scope __device__ array of

@@ -482,6 +482,39 @@ void launch_multi_iteration(const MultiArgs &a, int parity, hipStream_t s)
     hipLaunchKernelGGL(k_mdot<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 0);
 }
 
+template <typename T>
+__global__ void k_msentinel(T *v, size_t total)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = sent_value(T());
+}
+
+template <typename T>
+void launch_multi_spmm_only(const MultiArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mspmv<T>, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, (const double *)a.p0);
+}
+
+template <typename T>
+void launch_multi_fwd_only(const MultiArgs &a, hipStream_t s)
+{
+    launch_msweep<T, true>(a, (const double *)a.p0, 0, s);
+}
+
+template <typename T>
+void launch_multi_bwd_only(const MultiArgs &a, hipStream_t s)
+{
+    launch_msweep<T, false>(a, (const double *)nullptr, 0, s);
+}
+
+template <typename T>
+void launch_multi_sentinels(const MultiArgs &a, bool y, bool z, hipStream_t s)
+{
+    const size_t total = (size_t)a.n * a.K;         // the dummy row n keeps its zeros
+    if (y) hipLaunchKernelGGL(k_msentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.y, total);
+    if (z) hipLaunchKernelGGL(k_msentinel<T>, dim3(2048), dim3(256), 0, s, (T *)a.z, total);
+}
+
 void launch_multi_residual(const MultiArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mresidual, dim3(a.reduce_blocks), dim3(kMB), 0, s, a, 0);
@@ -500,5 +533,13 @@ template void launch_multi_init<float>(const MultiArgs &, hipStream_t);
 template void launch_multi_init<double>(const MultiArgs &, hipStream_t);
 template void launch_multi_iteration<float>(const MultiArgs &, int, hipStream_t);
 template void launch_multi_iteration<double>(const MultiArgs &, int, hipStream_t);
+template void launch_multi_spmm_only<float>(const MultiArgs &, hipStream_t);
+template void launch_multi_spmm_only<double>(const MultiArgs &, hipStream_t);
+template void launch_multi_fwd_only<float>(const MultiArgs &, hipStream_t);
+template void launch_multi_fwd_only<double>(const MultiArgs &, hipStream_t);
+template void launch_multi_bwd_only<float>(const MultiArgs &, hipStream_t);
+template void launch_multi_bwd_only<double>(const MultiArgs &, hipStream_t);
+template void launch_multi_sentinels<float>(const MultiArgs &, bool, bool, hipStream_t);
+template void launch_multi_sentinels<double>(const MultiArgs &, bool, bool, hipStream_t);
 
 }  // namespace rc

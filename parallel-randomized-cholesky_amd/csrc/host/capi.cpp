@@ -150,10 +150,12 @@ static void reset_system(rc_handle_t h)
     h->bstats.clear();
     // multi-RHS state belongs to the previous system: its vectors were sized
     // for that system's n
+    h->base_dev = false;
     h->multi_ready = false;
     h->m_k = 0;
     h->m_n = 0;
     h->m_solved = false;
+    h->m_ran = false;
     // pools are sized from the new system's nnz at its first factor (the
     // device buffers themselves are grow-only and stay allocated)
     h->pool_cap = 0;
@@ -258,6 +260,9 @@ static int rc_get_size_impl(rc_handle_t h, int64_t *n, int64_t *nnz)
 }
 
 // ------------------------------------------------------------------ ordering
+static int gnn_setup(rc_handle_t h, bool keep);
+static GnnArgs gnn_args(rc_handle_t h);
+
 static int rc_order_impl(rc_handle_t h, int method, uint64_t seed)
 {
     RC_CHECK_HANDLE();
@@ -280,10 +285,17 @@ static int rc_order_impl(rc_handle_t h, int method, uint64_t seed)
     case RC_ORDER_LEARNED:
         if (h->gnn_w.empty() || h->host_only) p = nnz_sort_perm(h->base, nv, seed, skip);      // fallback
         else {
-            std::vector<float> sc((size_t)nv);
-            int rc = rc_gnn_scores(h, h->gnn_w.data(), sc.data());
+            // network scores and the Plackett-Luce draw stay on the device
+            RC_HIP(hipSetDevice(h->opt.device));
+            int rc = gnn_setup(h, false);
             if (rc != RC_OK) return rc;
-            p = order_from_scores(sc, seed);
+            RC_HIP(hipMemcpyAsync(h->g_w.p, h->gnn_w.data(), (size_t)kGnnParams * sizeof(float), hipMemcpyHostToDevice,
+                                  h->stream));
+            GnnArgs g = gnn_args(h);
+            gnn_forward(g, h->g_w.as<float>(), h->gnn_keep, h->g_s.as<float>(), h->stream);
+            RC_HIP(hipGetLastError());
+            rc = learned_order_device(h, h->g_s.as<float>(), nv, seed, p);
+            if (rc != RC_OK) return rc;
         }
         break;
     default: return fail(h, RC_ERR_ARG, "unknown ordering");
@@ -482,44 +494,17 @@ static int prepare(rc_handle_t h)
 {
     if (h->prepared) return RC_OK;
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
-    int rc0 = prepare_host(h);
-    if (rc0 != RC_OK) return rc0;
-
-    // factor input: strict lower part
-    std::vector<int> lp, lr;
-    std::vector<double> lw;
-    int rc = lower_part(h->fac, lp, lr, lw);
-    if (rc != RC_OK) return fail(h, rc, "lower part too large");
-    h->nnz_low = (int64_t)lr.size();
-    // dep[r] = strict-lower entries (r, j), j < r: the reference's count of
-    // strict-upper entries of column r (pre_process.hpp:474-496)
-    std::vector<int> dep((size_t)h->nf, 0);
-    for (int v : lr) dep[(size_t)v]++;
-    RC_HIP(h->dep_init.ensure(dep.size() * sizeof(int)));
-    RC_HIP(hipMemcpyAsync(h->dep_init.p, dep.data(), dep.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    RC_HIP(h->lo_ptr.ensure(lp.size() * sizeof(int)));
-    RC_HIP(h->lo_row.ensure(std::max<size_t>(lr.size(), 1) * sizeof(int)));
-    RC_HIP(h->lo_w.ensure(std::max<size_t>(lw.size(), 1) * sizeof(double)));
-    RC_HIP(hipMemcpyAsync(h->lo_ptr.p, lp.data(), lp.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    if (!lr.empty()) {
-        RC_HIP(hipMemcpyAsync(h->lo_row.p, lr.data(), lr.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-        RC_HIP(hipMemcpyAsync(h->lo_w.p, lw.data(), lw.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->kind < 0) return fail(h, RC_ERR_STATE, "no matrix loaded");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (h->perm.empty()) {
+        h->perm.resize((size_t)h->n_user);
+        for (int i = 0; i < h->n_user; ++i) h->perm[(size_t)i] = i;
     }
-
-    // PCG system (CSR == CSC), int32
+    // permuted matrix, ground append / trim, lower slice and dependency
+    // counts, all on the device (prep_device.cpp)
+    int rc = prepare_device(h);
+    if (rc != RC_OK) return rc;
     const int n = h->ns;
-    std::vector<int> ap((size_t)n + 1);
-    for (int i = 0; i <= n; ++i) ap[(size_t)i] = (int)h->sys.ptr[(size_t)i];
-    RC_HIP(h->a_ptr.ensure(ap.size() * sizeof(int)));
-    RC_HIP(h->a_ind.ensure(std::max<size_t>(h->sys.ind.size(), 1) * sizeof(int)));
-    RC_HIP(h->a_val.ensure(std::max<size_t>(h->sys.val.size(), 1) * sizeof(double)));
-    RC_HIP(hipMemcpyAsync(h->a_ptr.p, ap.data(), ap.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    if (h->sys.nnz() > 0) {
-        RC_HIP(hipMemcpyAsync(h->a_ind.p, h->sys.ind.data(), h->sys.ind.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-        RC_HIP(hipMemcpyAsync(h->a_val.p, h->sys.val.data(), h->sys.val.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    }
-    RC_HIP(h->perm_d.ensure((size_t)std::max(n, 1) * sizeof(int)));
-    RC_HIP(hipMemcpyAsync(h->perm_d.p, h->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, h->stream));
     if (h->nsys > 0) {
         // system of every row (user and factored numbering) and the
         // reduction-block table: system s owns q in [sys_off[s], sys_off[s+1])
@@ -557,6 +542,7 @@ static int prepare(rc_handle_t h)
     h->prepared = true;
     h->factored = false;
     h->rhs_set = false;
+    h->prepare_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     invalidate_graph(h);
     return RC_OK;
 }
@@ -1104,13 +1090,16 @@ static int rc_factor_impl(rc_handle_t h, rc_factor_stats *st)
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
     RC_HIP(hipSetDevice(h->opt.device));
+    const bool had = h->prepared;
     int rc = prepare(h);
     if (rc != RC_OK) return rc;
+    const double prep_ms = had ? 0.0 : h->prepare_ms;
     std::memset(&h->fstats, 0, sizeof(h->fstats));
     double fms = 0.0, ams = 0.0;
     int regrow = 0;
     h->factored = false;
     h->m_solved = false;
+    h->m_ran = false;
     invalidate_graph(h);
     rc = run_elimination(h, &fms, &regrow);
     if (rc != RC_OK) return rc;
@@ -1124,6 +1113,108 @@ static int rc_factor_impl(rc_handle_t h, rc_factor_stats *st)
     h->fstats.assemble_ms = ams;
     h->fstats.pool_regrowths = regrow;
     h->fstats.bytes_factor = factor_bytes(h);
+    h->fstats.prepare_ms = prep_ms;
+    h->factored = true;
+    if (st) *st = h->fstats;
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------ IC(0) factor
+// The context preconditioner: incomplete Cholesky IC(0) of the prepared
+// matrix (ic0.hip) written into the elimination's output layout, then the
+// same assembly, so every solve entry point runs with it unchanged.
+static int rc_factor_ic0_impl(rc_handle_t h, rc_factor_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    RC_HIP(hipSetDevice(h->opt.device));
+    const bool had = h->prepared;
+    int rc = prepare(h);
+    if (rc != RC_OK) return rc;
+    const double prep_ms = had ? 0.0 : h->prepare_ms;
+    std::memset(&h->fstats, 0, sizeof(h->fstats));
+    h->factored = false;
+    h->m_solved = false;
+    h->m_ran = false;
+    invalidate_graph(h);
+    hipStream_t s = h->stream;
+    const int nf = h->nf;
+    const int *fp = h->fac_in_f ? h->f_ptr.as<int>() : h->a_ptr.as<int>();
+    const int *fi = h->fac_in_f ? h->f_ind.as<int>() : h->a_ind.as<int>();
+    const double *fv = h->fac_in_f ? h->f_val.as<double>() : h->a_val.as<double>();
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    RC_HIP(hipEventRecord(e0, s));
+    // rows of L = upper slices of the columns (dep_init = their lengths)
+    RC_HIP(h->ic_up_ptr.ensure(((size_t)nf + 1) * sizeof(int)));
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(nf + 3) * sizeof(int) + 64));
+    scan_exclusive(h->dep_init.as<int>(), h->ic_up_ptr.as<int>(), nf, h->tmp_scan.as<int>(), s);
+    int nup = 0;
+    RC_HIP(hipMemcpyAsync(&nup, h->ic_up_ptr.as<int>() + nf, sizeof(int), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipStreamSynchronize(s));
+    if ((int64_t)nup != h->nnz_low) return fail(h, RC_ERR_STATE, "IC(0): upper and lower slices differ in size");
+    RC_HIP(h->ic_up_ind.ensure((size_t)std::max(nup, 1) * sizeof(int)));
+    RC_HIP(h->ic_up_val.ensure((size_t)std::max(nup, 1) * sizeof(double)));
+    RC_HIP(h->ic_l.ensure((size_t)std::max(nup, 1) * sizeof(double)));
+    RC_HIP(h->ic_diag.ensure((size_t)nf * sizeof(double)));
+    launch_ic0_upper(nf, fp, fi, fv, h->ic_up_ptr.as<int>(), h->ic_up_ind.as<int>(), h->ic_up_val.as<double>(),
+                     h->ic_diag.as<double>(), s);
+    // the elimination's output arrays
+    if (h->pool_cap == 0) {
+        h->pool_cap = 1024;
+        h->scratch_cap = 8192;
+    }
+    h->out_cap = std::max<long long>(h->out_cap, h->nnz_low + 16);
+    rc = alloc_factor_work(h);
+    if (rc != RC_OK) return rc;
+    RC_HIP(hipMemsetAsync(h->ctr.p, 0, kCtrCount * sizeof(unsigned long long), s));
+    launch_ptr_to_start_cnt(nf, h->lo_ptr.as<int>(), h->out_start.as<long long>(), h->out_cnt.as<int>(), s);
+    if (h->nnz_low)
+        RC_HIP(hipMemcpyAsync(h->out_row.p, h->lo_row.p, (size_t)h->nnz_low * sizeof(int), hipMemcpyDeviceToDevice, s));
+    launch_fill_sentinel<double>(h->diag64.as<double>(), nf, s);
+    Ic0Args a;
+    a.n = nf;
+    a.up_ptr = h->ic_up_ptr.as<int>();
+    a.up_ind = h->ic_up_ind.as<int>();
+    a.up_val = h->ic_up_val.as<double>();
+    a.diag_a = h->ic_diag.as<double>();
+    a.l = h->ic_l.as<double>();
+    a.d = h->diag64.as<double>();
+    a.level = h->level.as<int>();
+    unsigned int *cw = reinterpret_cast<unsigned int *>(h->ctr.as<unsigned long long>() + kCtrCount - 3);
+    a.claim = cw;
+    a.status = cw + 2;
+    a.spin_ticks = h->spin_ticks;
+    launch_ic0(a, std::max(64, h->num_blocks * 4), s);
+    launch_ic0_scatter(nf, a.up_ptr, a.up_ind, a.l, h->lo_ptr.as<int>(), h->lo_row.as<int>(),
+                       h->out_val.as<double>(), s);
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipEventRecord(e1, s));
+    unsigned int status = 0;
+    RC_HIP(hipMemcpyAsync(&status, a.status, sizeof(status), hipMemcpyDeviceToHost, s));
+    unsigned long long top = (unsigned long long)h->nnz_low;
+    RC_HIP(hipMemcpyAsync(h->ctr.as<unsigned long long>() + kCtrOutTop, &top, sizeof(top), hipMemcpyHostToDevice, s));
+    RC_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (status & 1u) return fail(h, RC_ERR_TIMEOUT, "IC(0) row wait timed out");
+    h->prec = h->opt.factor_precision;
+    double ams = 0.0;
+    rc = (h->prec == 32) ? assemble<float>(h, &ams) : assemble<double>(h, &ams);
+    if (rc != RC_OK) return rc;
+    h->fstats.n = nf;
+    h->fstats.nnz_lower = h->nnz_low;
+    h->fstats.nnz_factor = h->nnz_low + nf;
+    h->fstats.nlevels = h->nlevels;
+    h->fstats.factor_ms = ms;
+    h->fstats.assemble_ms = ams;
+    h->fstats.prepare_ms = prep_ms;
+    // IC(0) reads the lower slice once and writes L once: 12 nnz_low + 4 (n+1)
+    // in, (4+8) nnz_low + 8 n out
+    h->fstats.bytes_factor = 24.0 * (double)h->nnz_low + 12.0 * (double)nf + 4.0;
     h->factored = true;
     if (st) *st = h->fstats;
     return RC_OK;
@@ -1497,40 +1588,6 @@ static int rc_set_reference_solution_impl(rc_handle_t h, const double *x_ref)
     return RC_OK;
 }
 
-static int build_graph(rc_handle_t h)
-{
-    if (h->graph && h->graph_prec == h->prec) return RC_OK;
-    invalidate_graph(h);
-    SolveArgs a = solve_args(h);
-    hipGraph_t g;
-    RC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    if (h->nsys > 0) {
-        BatchArgs b = batch_args(h);
-        if (h->prec == 32) {
-            launch_batch_iteration<float>(a, b, 0, h->stream);
-            launch_batch_iteration<float>(a, b, 1, h->stream);
-        } else {
-            launch_batch_iteration<double>(a, b, 0, h->stream);
-            launch_batch_iteration<double>(a, b, 1, h->stream);
-        }
-    } else if (h->prec == 32) {
-        launch_pcg_iteration<float>(a, 0, h->stream);
-        launch_pcg_iteration<float>(a, 1, h->stream);
-    } else {
-        launch_pcg_iteration<double>(a, 0, h->stream);
-        launch_pcg_iteration<double>(a, 1, h->stream);
-    }
-    RC_HIP(hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        h->graph = nullptr;
-        return fail(h, RC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    }
-    h->graph_prec = h->prec;
-    return RC_OK;
-}
-
 static int set_rhs_device(rc_handle_t h, const double *b_user)
 {
     int rc = alloc_solve(h);
@@ -1539,130 +1596,12 @@ static int set_rhs_device(rc_handle_t h, const double *b_user)
     launch_gather(h->bu.as<double>(), h->idx_user.as<int>(), h->b.as<double>(), h->ns, h->stream);
     RC_HIP(hipGetLastError());
     h->rhs_set = true;
+    h->rhs_mode = 1;
     return RC_OK;
 }
 
-static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
-{
-    int rc = alloc_solve(h);
-    if (rc != RC_OK) return rc;
-    hipStream_t s = h->stream;
-    PcgScalars sc;
-    std::memset(&sc, 0, sizeof(sc));
-    sc.tol = tol;
-    sc.maxit = h->opt.maxit;
-    sc.stop_rule = h->opt.stop_rule;
-    hipEvent_t e0, e1, e2;
-    RC_HIP(hipEventCreate(&e0));
-    RC_HIP(hipEventCreate(&e1));
-    RC_HIP(hipEventCreate(&e2));
-    rc = build_graph(h);
-    if (rc != RC_OK) return rc;
-    SolveArgs a = solve_args(h);
-    RC_HIP(hipEventRecord(e0, s));
-    RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
-    if (h->nsys > 0) {
-        BatchArgs b = batch_args(h);
-        RC_HIP(hipMemsetAsync(h->bt_state.p, 0, batch_state_bytes(h->nsys, h->nblk), s));
-        if (h->prec == 32) launch_batch_init<float>(a, b, s);
-        else launch_batch_init<double>(a, b, s);
-    } else if (h->prec == 32) {
-        launch_pcg_init<float>(a, s);
-    } else {
-        launch_pcg_init<double>(a, s);
-    }
-    RC_HIP(hipGetLastError());
-    RC_HIP(hipEventRecord(e1, s));
-    // iterations: 2 per graph launch; poll the done flag every 4 launches
-    int *done_h = nullptr;
-    RC_HIP(hipHostMalloc((void **)&done_h, sizeof(int) * 2, hipHostMallocDefault));
-    const int per_poll = 4;
-    int launched = 0;
-    const int max_launch = (h->opt.maxit + 1) / 2 + per_poll;
-    done_h[0] = 0;
-    while (launched < max_launch) {
-        for (int t = 0; t < per_poll && launched < max_launch; ++t, ++launched) {
-            hipError_t e = hipGraphLaunch(h->graph, s);
-            if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e)); }
-        }
-        hipError_t e = hipMemcpyAsync(done_h, (char *)h->scal.p + offsetof(PcgScalars, done), sizeof(int),
-                                      hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
-        if (done_h[0]) break;
-    }
-    (void)hipHostFree(done_h);
-    RC_HIP(hipEventRecord(e2, s));
-    if (h->nsys > 0) launch_batch_residual(a, batch_args(h), s);
-    else launch_residual(a, s);
-    RC_HIP(hipMemcpyAsync(&sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, s));
-    std::vector<char> bt;
-    if (h->nsys > 0) {
-        bt.resize(batch_state_bytes(h->nsys, h->nblk));
-        RC_HIP(hipMemcpyAsync(bt.data(), h->bt_state.p, bt.size(), hipMemcpyDeviceToHost, s));
-    }
-    RC_HIP(hipStreamSynchronize(s));
-    float ms_setup = 0.f, ms_pcg = 0.f;
-    (void)hipEventElapsedTime(&ms_setup, e0, e1);
-    (void)hipEventElapsedTime(&ms_pcg, e1, e2);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    if (st) {
-        st->iterations = sc.iter;
-        st->converged = (sc.status == 0 && sc.done && sc.iter < h->opt.maxit) ? 1 : 0;
-        if (sc.status == 0 && sc.done && sc.iter == h->opt.maxit) {
-            bool met = h->opt.stop_rule == RC_STOP_REFERENCE ? (std::sqrt(sc.rz) / std::sqrt(sc.rz0) <= tol)
-                                                              : (std::sqrt(sc.rr) / std::sqrt(sc.bb) <= tol);
-            st->converged = met ? 1 : 0;
-        }
-        st->rel_res_true = sc.bnorm2 > 0.0 ? std::sqrt(sc.res2) / std::sqrt(sc.bnorm2) : 0.0;
-        st->rel_res_precond = sc.rz0 > 0.0 ? std::sqrt(std::max(sc.rz, 0.0)) / std::sqrt(sc.rz0) : 0.0;
-        st->pcg_ms = ms_pcg;
-        st->setup_ms = ms_setup;
-        st->err_vs_oracle = -1.0;
-    }
-    bool any_brk = false;
-    if (h->nsys > 0) {
-        // per-system results; the summary reports the worst system
-        BatchArgs hb;
-        hb.nsys = h->nsys;
-        hb.nblk = h->nblk;
-        batch_carve(hb, bt.data());
-        h->bstats.assign((size_t)h->nsys, rc_solve_stats{});
-        int max_it = 0, all_conv = 1;
-        double worst_true = 0.0, worst_pre = 0.0;
-        for (int q = 0; q < h->nsys; ++q) {
-            rc_solve_stats &o = h->bstats[(size_t)q];
-            o.iterations = hb.iter[q];
-            const double pre = hb.rz0[q] > 0.0 ? std::sqrt(std::max(hb.rz[q], 0.0)) / std::sqrt(hb.rz0[q]) : 0.0;
-            const double tr = hb.bn2[q] > 0.0 ? std::sqrt(hb.res2[q]) / std::sqrt(hb.bn2[q]) : 0.0;
-            const bool met = h->opt.stop_rule == RC_STOP_REFERENCE
-                                 ? (hb.rz0[q] <= tol || pre <= tol)
-                                 : (hb.bb[q] <= 0.0 || std::sqrt(hb.rr[q]) / std::sqrt(hb.bb[q]) <= tol);
-            o.converged = (sc.status == 0 && !hb.brk[q] && hb.done[q] && met) ? 1 : 0;
-            o.rel_res_true = tr;
-            o.rel_res_precond = pre;
-            o.pcg_ms = ms_pcg;
-            o.setup_ms = ms_setup;
-            o.err_vs_oracle = -1.0;
-            any_brk = any_brk || hb.brk[q];
-            max_it = std::max(max_it, o.iterations);
-            all_conv &= o.converged;
-            worst_true = std::max(worst_true, tr);
-            worst_pre = std::max(worst_pre, pre);
-        }
-        if (st) {
-            st->iterations = max_it;
-            st->converged = all_conv;
-            st->rel_res_true = worst_true;
-            st->rel_res_precond = worst_pre;
-        }
-    }
-    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
-    if ((sc.status & kPcgBreakdown) || any_brk) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown (p'Ap <= 0)");
-    return RC_OK;
-}
+// the single-system / batch PCG run (defined with the other PCG runs below)
+static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st);
 
 static int fetch_solution(rc_handle_t h, double *x_user)
 {
@@ -1849,83 +1788,431 @@ static int rc_set_rhs_multi_impl(rc_handle_t h, int k, const double *B)
     h->m_k = k;
     h->m_n = (int)n;
     h->m_solved = false;
+    h->m_ran = false;
+    h->rhs_mode = 2;
     return RC_OK;
 }
 
-static int rc_solve_multi_resident_impl(rc_handle_t h, double tol, rc_solve_stats *st)
+// ------------------------------------------------------------------ PCG runs
+// One run is the PCG of one handle in one of two modes: the single-RHS solve
+// (one system, or one right-hand side per system of a batch) or the
+// multi-RHS solve.  A run is split into
+//   begin: scalars uploaded and the init kernels launched on the handle's stream;
+//   pairs: two iterations at a time, captured into a hipGraph (cached per
+//          handle and mode for a run of one handle; one graph holding every
+//          member's pair for a group of handles);
+//   end:   the true residual and the statistics.
+// Kernels of a finished run (done flag set) return at once, so the pairs of
+// a group keep launching until every member is done.
+enum { kRunSingle = 1, kRunMulti = 2 };
+
+static int run_state_ok(rc_handle_t h, int mode)
 {
-    RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
-    if (!(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad tolerance");
+    if (mode == kRunSingle) {
+        if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and rc_set_rhs must precede");
+        return RC_OK;
+    }
     if (!h->factored || h->m_k <= 0) return fail(h, RC_ERR_STATE, "rc_factor and rc_set_rhs_multi must precede");
     if (h->nsys > 0) return fail(h, RC_ERR_STATE, "a batch is solved with rc_solve_batch (one RHS per system)");
     if ((size_t)h->ns != (size_t)h->n_user || h->m_n != h->n_user)
         return fail(h, RC_ERR_STATE, "right-hand sides do not match the system");
-    RC_HIP(hipSetDevice(h->opt.device));
+    return RC_OK;
+}
+
+static int run_begin(rc_handle_t h, int mode, double tol, MultiArgs &ma)
+{
     hipStream_t s = h->stream;
-    hipEvent_t e0, e1;
-    RC_HIP(hipEventCreate(&e0));
-    RC_HIP(hipEventCreate(&e1));
-    RC_HIP(hipEventRecord(e0, s));
+    if (mode == kRunSingle) {
+        int rc = alloc_solve(h);
+        if (rc != RC_OK) return rc;
+        PcgScalars sc;
+        std::memset(&sc, 0, sizeof(sc));
+        sc.tol = tol;
+        sc.maxit = h->opt.maxit;
+        sc.stop_rule = h->opt.stop_rule;
+        SolveArgs a = solve_args(h);
+        RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
+        if (h->nsys > 0) {
+            BatchArgs b = batch_args(h);
+            RC_HIP(hipMemsetAsync(h->bt_state.p, 0, batch_state_bytes(h->nsys, h->nblk), s));
+            if (h->prec == 32) launch_batch_init<float>(a, b, s);
+            else launch_batch_init<double>(a, b, s);
+        } else if (h->prec == 32) {
+            launch_pcg_init<float>(a, s);
+        } else {
+            launch_pcg_init<double>(a, s);
+        }
+        RC_HIP(hipGetLastError());
+        return RC_OK;
+    }
     int rc = h->prec == 32 ? build_multi<float>(h) : build_multi<double>(h);
     if (rc != RC_OK) return rc;
-    const int k = h->m_k;
-    MultiArgs a;
-    rc = multi_args(h, k, a);
+    rc = multi_args(h, h->m_k, ma);
     if (rc != RC_OK) return rc;
-    const size_t n = (size_t)h->ns;
     MultiScalars sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.tol = tol;
     sc.maxit = h->opt.maxit;
     sc.stop_rule = h->opt.stop_rule;
     RC_HIP(hipMemcpyAsync(h->m_sc.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
-    launch_multi_gather(h->m_B.as<double>(), h->idx_user.as<int>(), (int)n, k, a.K, h->m_b.as<double>(), s);
-    if (h->prec == 32) launch_multi_init<float>(a, s); else launch_multi_init<double>(a, s);
-    int *done_h = nullptr;
-    RC_HIP(hipHostMalloc((void **)&done_h, sizeof(int) * 2, hipHostMallocDefault));
-    done_h[0] = 0;
-    int it = 0;
-    while (it < h->opt.maxit + 2) {
-        for (int t = 0; t < 8; ++t, ++it) {
-            if (h->prec == 32) launch_multi_iteration<float>(a, it & 1, s);
-            else launch_multi_iteration<double>(a, it & 1, s);
-        }
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(done_h, (char *)h->m_sc.p + offsetof(MultiScalars, all_done),
-                                                sizeof(int), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) { (void)hipHostFree(done_h); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
-        if (done_h[0]) break;
-    }
-    (void)hipHostFree(done_h);
-    launch_multi_residual(a, s);
-    RC_HIP(hipEventRecord(e1, s));
-    RC_HIP(hipMemcpyAsync(&sc, h->m_sc.p, sizeof(sc), hipMemcpyDeviceToHost, s));
-    RC_HIP(hipStreamSynchronize(s));
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    h->m_solved = true;
-    int any_break = 0;
-    for (int q = 0; q < k; ++q) {
-        any_break |= sc.breakdown[q];
-        if (!st) continue;
-        st[q].iterations = sc.iter[q];
-        bool met = h->opt.stop_rule == RC_STOP_REFERENCE
-                       ? (sc.rz0[q] <= tol || std::sqrt(sc.rz[q]) / std::sqrt(sc.rz0[q]) <= tol)
-                       : (sc.bb[q] <= 0.0 || std::sqrt(sc.rr[q]) / std::sqrt(sc.bb[q]) <= tol);
-        st[q].converged = (sc.done[q] && !sc.breakdown[q] && (sc.iter[q] < h->opt.maxit || met)) ? 1 : 0;
-        st[q].rel_res_true = sc.bb[q] > 0.0 ? std::sqrt(sc.res2[q]) / std::sqrt(sc.bb[q]) : 0.0;
-        st[q].rel_res_precond = sc.rz0[q] > 0.0 ? std::sqrt(std::max(sc.rz[q], 0.0)) / std::sqrt(sc.rz0[q]) : 0.0;
-        st[q].pcg_ms = ms;
-        st[q].setup_ms = 0.0;
-        st[q].err_vs_oracle = -1.0;
-    }
-    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
-    if (any_break) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown in at least one right-hand side");
+    launch_multi_gather(h->m_B.as<double>(), h->idx_user.as<int>(), ma.n, ma.k, ma.K, h->m_b.as<double>(), s);
+    if (h->prec == 32) launch_multi_init<float>(ma, s);
+    else launch_multi_init<double>(ma, s);
+    RC_HIP(hipGetLastError());
     return RC_OK;
+}
+
+// Two iterations of the run on stream s (parities 0 and 1).
+static void run_pair(rc_handle_t h, int mode, const MultiArgs &ma, hipStream_t s)
+{
+    if (mode == kRunMulti) {
+        for (int par = 0; par < 2; ++par) {
+            if (h->prec == 32) launch_multi_iteration<float>(ma, par, s);
+            else launch_multi_iteration<double>(ma, par, s);
+        }
+        return;
+    }
+    SolveArgs a = solve_args(h);
+    for (int par = 0; par < 2; ++par) {
+        if (h->nsys > 0) {
+            BatchArgs b = batch_args(h);
+            if (h->prec == 32) launch_batch_iteration<float>(a, b, par, s);
+            else launch_batch_iteration<double>(a, b, par, s);
+        } else if (h->prec == 32) {
+            launch_pcg_iteration<float>(a, par, s);
+        } else {
+            launch_pcg_iteration<double>(a, par, s);
+        }
+    }
+}
+
+static const int *run_done_flag(rc_handle_t h, int mode)
+{
+    if (mode == kRunMulti) return reinterpret_cast<const int *>(h->m_sc.as<char>() + offsetof(MultiScalars, all_done));
+    return reinterpret_cast<const int *>(h->scal.as<char>() + offsetof(PcgScalars, done));
+}
+
+// The cached graph of a run of one handle: two iterations, rebuilt when the
+// buffers it was captured with change.
+static int run_graph(rc_handle_t h, int mode, const MultiArgs &ma, hipGraphExec_t *out)
+{
+    const void *key = mode == kRunMulti ? h->m_x.p : h->y.p;
+    hipGraphExec_t *slot = mode == kRunMulti ? &h->m_graph : &h->graph;
+    const int kk = mode == kRunMulti ? ma.K : 0;
+    if (*slot && h->graph_key[mode] == key && h->graph_prec_of[mode] == h->prec && h->graph_K[mode] == kk) {
+        *out = *slot;
+        return RC_OK;
+    }
+    if (*slot) (void)hipGraphExecDestroy(*slot);
+    *slot = nullptr;
+    hipGraph_t g;
+    RC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    run_pair(h, mode, ma, h->stream);
+    RC_HIP(hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *slot = nullptr;
+        return fail(h, RC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    h->graph_key[mode] = key;
+    h->graph_prec_of[mode] = h->prec;
+    h->graph_K[mode] = kk;
+    *out = *slot;
+    return RC_OK;
+}
+
+// Residual and statistics of a finished run; st holds 1 entry (single mode:
+// the batch's worst system for a batch) or k entries (multi mode).  Sets
+// *broke when a breakdown ended the run (some RHS / system of it).
+static int run_end(rc_handle_t h, int mode, double tol, const MultiArgs &ma, float ms_setup, float ms_pcg,
+                   rc_solve_stats *st, bool *broke)
+{
+    hipStream_t s = h->stream;
+    *broke = false;
+    if (mode == kRunMulti) {
+        launch_multi_residual(ma, s);
+        MultiScalars sc;
+        RC_HIP(hipMemcpyAsync(&sc, h->m_sc.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipStreamSynchronize(s));
+        h->m_solved = true;
+        h->m_ran = true;
+        const int k = h->m_k;
+        h->m_stats.assign((size_t)k, rc_solve_stats{});
+        for (int q = 0; q < k; ++q) {
+            rc_solve_stats &o = h->m_stats[(size_t)q];
+            *broke = *broke || sc.breakdown[q];
+            o.iterations = sc.iter[q];
+            const bool met = h->opt.stop_rule == RC_STOP_REFERENCE
+                                 ? (sc.rz0[q] <= tol || std::sqrt(sc.rz[q]) / std::sqrt(sc.rz0[q]) <= tol)
+                                 : (sc.bb[q] <= 0.0 || std::sqrt(sc.rr[q]) / std::sqrt(sc.bb[q]) <= tol);
+            o.converged = (sc.done[q] && !sc.breakdown[q] && (sc.iter[q] < h->opt.maxit || met)) ? 1 : 0;
+            o.rel_res_true = sc.bb[q] > 0.0 ? std::sqrt(sc.res2[q]) / std::sqrt(sc.bb[q]) : 0.0;
+            o.rel_res_precond = sc.rz0[q] > 0.0 ? std::sqrt(std::max(sc.rz[q], 0.0)) / std::sqrt(sc.rz0[q]) : 0.0;
+            o.pcg_ms = ms_pcg;
+            o.setup_ms = ms_setup;
+            o.err_vs_oracle = -1.0;
+        }
+        if (st) std::memcpy(st, h->m_stats.data(), (size_t)k * sizeof(rc_solve_stats));
+        if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
+        if (*broke) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown in at least one right-hand side");
+        return RC_OK;
+    }
+    SolveArgs a = solve_args(h);
+    if (h->nsys > 0) launch_batch_residual(a, batch_args(h), s);
+    else launch_residual(a, s);
+    PcgScalars sc;
+    RC_HIP(hipMemcpyAsync(&sc, h->scal.p, sizeof(sc), hipMemcpyDeviceToHost, s));
+    std::vector<char> bt;
+    if (h->nsys > 0) {
+        bt.resize(batch_state_bytes(h->nsys, h->nblk));
+        RC_HIP(hipMemcpyAsync(bt.data(), h->bt_state.p, bt.size(), hipMemcpyDeviceToHost, s));
+    }
+    RC_HIP(hipStreamSynchronize(s));
+    rc_solve_stats o{};
+    o.iterations = sc.iter;
+    o.converged = (sc.status == 0 && sc.done && sc.iter < h->opt.maxit) ? 1 : 0;
+    if (sc.status == 0 && sc.done && sc.iter == h->opt.maxit) {
+        const bool met = h->opt.stop_rule == RC_STOP_REFERENCE ? (std::sqrt(sc.rz) / std::sqrt(sc.rz0) <= tol)
+                                                                : (std::sqrt(sc.rr) / std::sqrt(sc.bb) <= tol);
+        o.converged = met ? 1 : 0;
+    }
+    o.rel_res_true = sc.bnorm2 > 0.0 ? std::sqrt(sc.res2) / std::sqrt(sc.bnorm2) : 0.0;
+    o.rel_res_precond = sc.rz0 > 0.0 ? std::sqrt(std::max(sc.rz, 0.0)) / std::sqrt(sc.rz0) : 0.0;
+    o.pcg_ms = ms_pcg;
+    o.setup_ms = ms_setup;
+    o.err_vs_oracle = -1.0;
+    bool any_brk = false;
+    if (h->nsys > 0) {
+        // per-system results; the summary reports the worst system
+        BatchArgs hb;
+        hb.nsys = h->nsys;
+        hb.nblk = h->nblk;
+        batch_carve(hb, bt.data());
+        h->bstats.assign((size_t)h->nsys, rc_solve_stats{});
+        int max_it = 0, all_conv = 1;
+        double worst_true = 0.0, worst_pre = 0.0;
+        for (int q = 0; q < h->nsys; ++q) {
+            rc_solve_stats &b = h->bstats[(size_t)q];
+            b.iterations = hb.iter[q];
+            const double pre = hb.rz0[q] > 0.0 ? std::sqrt(std::max(hb.rz[q], 0.0)) / std::sqrt(hb.rz0[q]) : 0.0;
+            const double tr = hb.bn2[q] > 0.0 ? std::sqrt(hb.res2[q]) / std::sqrt(hb.bn2[q]) : 0.0;
+            const bool met = h->opt.stop_rule == RC_STOP_REFERENCE
+                                 ? (hb.rz0[q] <= tol || pre <= tol)
+                                 : (hb.bb[q] <= 0.0 || std::sqrt(hb.rr[q]) / std::sqrt(hb.bb[q]) <= tol);
+            b.converged = (sc.status == 0 && !hb.brk[q] && hb.done[q] && met) ? 1 : 0;
+            b.rel_res_true = tr;
+            b.rel_res_precond = pre;
+            b.pcg_ms = ms_pcg;
+            b.setup_ms = ms_setup;
+            b.err_vs_oracle = -1.0;
+            any_brk = any_brk || hb.brk[q];
+            max_it = std::max(max_it, b.iterations);
+            all_conv &= b.converged;
+            worst_true = std::max(worst_true, tr);
+            worst_pre = std::max(worst_pre, pre);
+        }
+        o.iterations = max_it;
+        o.converged = all_conv;
+        o.rel_res_true = worst_true;
+        o.rel_res_precond = worst_pre;
+    }
+    if (st) *st = o;
+    *broke = (sc.status & kPcgBreakdown) || any_brk;
+    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "triangular solve wait timed out");
+    if (*broke) return fail(h, RC_ERR_BREAKDOWN, "PCG breakdown (p'Ap <= 0)");
+    return RC_OK;
+}
+
+// Runs the PCGs of `count` handles (modes[i]) together: every member's init
+// on its own stream, then graphs of two iterations of every member on the
+// first handle's stream until all are done, then every member's end.
+// st[i] points to member i's statistics (1 or k entries, or nullptr).
+// Returns the first member's error (every member is finished regardless);
+// a breakdown is reported only when nothing worse happened.
+static int run_group(int count, const rc_handle_t *hs, const int *modes, double tol, rc_solve_stats *const *st)
+{
+    rc_handle_t h0 = hs[0];
+    hipStream_t o = h0->stream;
+    std::vector<MultiArgs> ma((size_t)count);
+    std::vector<hipEvent_t> ev0((size_t)count, nullptr), ev1((size_t)count, nullptr);
+    hipEvent_t el0 = nullptr, el1 = nullptr, join = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool own_exec = false;
+    int *done_h = nullptr;
+    int rc = RC_OK;
+    auto cleanup = [&]() {
+        for (auto e : ev0) if (e) (void)hipEventDestroy(e);
+        for (auto e : ev1) if (e) (void)hipEventDestroy(e);
+        if (el0) (void)hipEventDestroy(el0);
+        if (el1) (void)hipEventDestroy(el1);
+        if (join) (void)hipEventDestroy(join);
+        if (own_exec && exec) (void)hipGraphExecDestroy(exec);
+        if (done_h) (void)hipHostFree(done_h);
+    };
+    auto hip_fail = [&](rc_handle_t h, hipError_t e, const char *what) {
+        cleanup();
+        return fail(h, e == hipErrorOutOfMemory ? RC_ERR_NOMEM : RC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < count; ++i) {
+        rc_handle_t h = hs[i];
+        if ((e = hipEventCreate(&ev0[(size_t)i])) != hipSuccess || (e = hipEventCreate(&ev1[(size_t)i])) != hipSuccess)
+            return hip_fail(h, e, "hipEventCreate");
+        if ((e = hipEventRecord(ev0[(size_t)i], h->stream)) != hipSuccess) return hip_fail(h, e, "hipEventRecord");
+        rc = run_begin(h, modes[i], tol, ma[(size_t)i]);
+        if (rc != RC_OK) {
+            if (i > 0) fail(h0, rc, std::string("group member ") + std::to_string(i) + ": " + h->err);
+            cleanup();
+            return rc;
+        }
+        if ((e = hipEventRecord(ev1[(size_t)i], h->stream)) != hipSuccess) return hip_fail(h, e, "hipEventRecord");
+        if (i > 0 && (e = hipStreamWaitEvent(o, ev1[(size_t)i], 0)) != hipSuccess) return hip_fail(h0, e, "hipStreamWaitEvent");
+    }
+    if (count == 1) {
+        rc = run_graph(h0, modes[0], ma[0], &exec);
+        if (rc != RC_OK) { cleanup(); return rc; }
+    } else {
+        hipGraph_t g;
+        if ((e = hipStreamBeginCapture(o, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(h0, e, "hipStreamBeginCapture");
+        for (int i = 0; i < count; ++i) run_pair(hs[i], modes[i], ma[(size_t)i], o);
+        if ((e = hipStreamEndCapture(o, &g)) != hipSuccess) return hip_fail(h0, e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { exec = nullptr; return hip_fail(h0, e, "hipGraphInstantiate"); }
+        own_exec = true;
+    }
+    if ((e = hipEventCreate(&el0)) != hipSuccess || (e = hipEventCreate(&el1)) != hipSuccess ||
+        (e = hipEventCreate(&join)) != hipSuccess)
+        return hip_fail(h0, e, "hipEventCreate");
+    if ((e = hipHostMalloc((void **)&done_h, sizeof(int) * (size_t)count, hipHostMallocDefault)) != hipSuccess)
+        return hip_fail(h0, e, "hipHostMalloc");
+    if ((e = hipEventRecord(el0, o)) != hipSuccess) return hip_fail(h0, e, "hipEventRecord");
+    // 2 iterations per graph launch; the done flags are read every 4 launches
+    const int per_poll = 4;
+    const int max_launch = (h0->opt.maxit + 1) / 2 + per_poll;
+    int maxit_all = 0;
+    for (int i = 0; i < count; ++i) maxit_all = std::max(maxit_all, hs[i]->opt.maxit);
+    const int launches = (maxit_all + 1) / 2 + per_poll;
+    (void)max_launch;
+    for (int launched = 0; launched < launches;) {
+        for (int t = 0; t < per_poll && launched < launches; ++t, ++launched)
+            if ((e = hipGraphLaunch(exec, o)) != hipSuccess) return hip_fail(h0, e, "hipGraphLaunch");
+        for (int i = 0; i < count; ++i)
+            if ((e = hipMemcpyAsync(done_h + i, run_done_flag(hs[i], modes[i]), sizeof(int), hipMemcpyDeviceToHost,
+                                    o)) != hipSuccess)
+                return hip_fail(h0, e, "hipMemcpyAsync");
+        if ((e = hipStreamSynchronize(o)) != hipSuccess) return hip_fail(h0, e, "hipStreamSynchronize");
+        bool all = true;
+        for (int i = 0; i < count; ++i) all = all && done_h[i];
+        if (all) break;
+    }
+    if ((e = hipEventRecord(el1, o)) != hipSuccess) return hip_fail(h0, e, "hipEventRecord");
+    if ((e = hipEventRecord(join, o)) != hipSuccess) return hip_fail(h0, e, "hipEventRecord");
+    (void)hipEventSynchronize(el1);
+    float ms_pcg = 0.f;
+    (void)hipEventElapsedTime(&ms_pcg, el0, el1);
+    int first = RC_OK;
+    for (int i = 0; i < count; ++i) {
+        rc_handle_t h = hs[i];
+        if (i > 0 && (e = hipStreamWaitEvent(h->stream, join, 0)) != hipSuccess) return hip_fail(h, e, "hipStreamWaitEvent");
+        float ms_setup = 0.f;
+        (void)hipEventElapsedTime(&ms_setup, ev0[(size_t)i], ev1[(size_t)i]);
+        bool broke = false;
+        const int r = run_end(h, modes[i], tol, ma[(size_t)i], ms_setup, ms_pcg, st[i], &broke);
+        if (r != RC_OK && (first == RC_OK || first == RC_ERR_BREAKDOWN)) {
+            first = r;
+            if (i > 0) fail(h0, r, std::string("group member ") + std::to_string(i) + ": " + h->err);
+        }
+    }
+    cleanup();
+    return first;
+}
+
+static int solve_resident(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    const int mode = kRunSingle;
+    rc_solve_stats *sts[1] = {st};
+    return run_group(1, &h, &mode, tol, sts);
+}
+
+static int rc_solve_multi_resident_impl(rc_handle_t h, double tol, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (!(tol >= 0.0)) return fail(h, RC_ERR_ARG, "bad tolerance");
+    int rc = run_state_ok(h, kRunMulti);
+    if (rc != RC_OK) return rc;
+    RC_HIP(hipSetDevice(h->opt.device));
+    const int mode = kRunMulti;
+    rc_solve_stats *sts[1] = {st};
+    return run_group(1, &h, &mode, tol, sts);
+}
+
+static int rc_set_solve_options_impl(rc_handle_t h, int stop_rule, int maxit)
+{
+    RC_CHECK_HANDLE();
+    if (stop_rule != RC_STOP_REFERENCE && stop_rule != RC_STOP_TRUE_RESIDUAL) return fail(h, RC_ERR_ARG, "unknown stop rule");
+    if (maxit < 0) return fail(h, RC_ERR_ARG, "maxit must be >= 0");
+    h->opt.stop_rule = stop_rule;
+    if (maxit > 0) h->opt.maxit = maxit;
+    return RC_OK;
+}
+
+static int rc_get_multi_stats_impl(rc_handle_t h, rc_solve_stats *st)
+{
+    RC_CHECK_HANDLE();
+    if (!h->m_solved || h->m_stats.size() != (size_t)h->m_k) return fail(h, RC_ERR_STATE, "no multi-RHS solve");
+    if (!st) return fail(h, RC_ERR_ARG, "null stats array");
+    std::memcpy(st, h->m_stats.data(), h->m_stats.size() * sizeof(rc_solve_stats));
+    return RC_OK;
+}
+
+static int rc_solve_group_impl(int count, const rc_handle_t *handles, double tol, rc_solve_stats *st)
+{
+    if (count <= 0 || !handles) return RC_ERR_ARG;
+    for (int i = 0; i < count; ++i)
+        if (!handles[i]) return RC_ERR_ARG;
+    rc_handle_t h0 = handles[0];
+    if (!(tol >= 0.0)) return fail(h0, RC_ERR_ARG, "bad tolerance");
+    std::vector<int> modes((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        rc_handle_t h = handles[i];
+        for (int j = 0; j < i; ++j)
+            if (handles[j] == h) return fail(h0, RC_ERR_ARG, "a handle appears twice in the group");
+        if (h->opt.device != h0->opt.device) return fail(h0, RC_ERR_ARG, "group members on different devices");
+        modes[(size_t)i] = h->rhs_mode == 2 ? kRunMulti : kRunSingle;
+        const int rc = run_state_ok(h, modes[(size_t)i]);
+        if (rc != RC_OK) {
+            if (i > 0) fail(h0, rc, std::string("group member ") + std::to_string(i) + ": " + h->err);
+            return rc;
+        }
+    }
+    RC_HIP_H(h0, hipSetDevice(h0->opt.device));
+    // member i's statistics: multi runs keep their per-RHS entries in the
+    // handle (rc_get_multi_stats); the group reports one summary per member
+    std::vector<std::vector<rc_solve_stats>> per((size_t)count);
+    std::vector<rc_solve_stats *> sp((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        per[(size_t)i].assign(modes[(size_t)i] == kRunMulti ? (size_t)handles[i]->m_k : 1, rc_solve_stats{});
+        sp[(size_t)i] = per[(size_t)i].data();
+    }
+    const int rc = run_group(count, handles, modes.data(), tol, sp.data());
+    if (st) {
+        for (int i = 0; i < count; ++i) {
+            const auto &v = per[(size_t)i];
+            rc_solve_stats o = v[0];
+            for (const auto &q : v) {
+                o.iterations = std::max(o.iterations, q.iterations);
+                o.converged = o.converged && q.converged;
+                o.rel_res_true = std::max(o.rel_res_true, q.rel_res_true);
+                o.rel_res_precond = std::max(o.rel_res_precond, q.rel_res_precond);
+            }
+            st[i] = o;
+        }
+    }
+    return rc;
 }
 
 static int rc_get_solution_multi_impl(rc_handle_t h, double *X)
@@ -1961,10 +2248,75 @@ static int rc_solve_multi_impl(rc_handle_t h, int k, const double *B, double *X,
 // ------------------------------------------------------------------ timing
 static const size_t kCalibBytes = (size_t)1 << 30;   // copy kernel: 1 GiB read + 1 GiB written
 
+// Multi-RHS kernels: the SpMM and the two sweeps of the last multi-RHS
+// system, k right-hand sides (bytes: SpMM 12 nnz + 4 (n+1) + 16 k n; sweep
+// (4+s) nnz(L) + 4 (n+1) + 2 s k n).
+static int time_multi_kernel(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
+{
+    if (!h->m_ran || h->m_k <= 0 || h->m_n != h->ns)
+        return fail(h, RC_ERR_STATE, "a multi-RHS solve must precede timing its kernels");
+    hipStream_t s = h->stream;
+    MultiArgs a;
+    int rc = multi_args(h, h->m_k, a);
+    if (rc != RC_OK) return rc;
+    MultiScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.maxit = 1 << 30;
+    for (int q = 0; q < kMaxRhs; ++q) { sc.rz[q] = 1.0; sc.rz0[q] = 1.0; }
+    RC_HIP(hipMemcpyAsync(h->m_sc.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1;
+    RC_HIP(hipEventCreate(&e0));
+    RC_HIP(hipEventCreate(&e1));
+    double total = 0.0;
+    const bool f32 = h->prec == 32;
+    for (int t = 0; t < reps; ++t) {
+        if (kernel == RC_KERNEL_MULTI_SPMM) {
+            RC_HIP(hipEventRecord(e0, s));
+            if (f32) launch_multi_spmm_only<float>(a, s); else launch_multi_spmm_only<double>(a, s);
+            RC_HIP(hipEventRecord(e1, s));
+        } else {
+            if (f32) launch_multi_sentinels<float>(a, true, true, s); else launch_multi_sentinels<double>(a, true, true, s);
+            launch_reset_counters(a.ctr_fwd, a.ctr_bwd, s);
+            if (kernel == RC_KERNEL_MULTI_TRSV_BWD) {
+                if (f32) launch_multi_fwd_only<float>(a, s); else launch_multi_fwd_only<double>(a, s);
+            }
+            RC_HIP(hipEventRecord(e0, s));
+            if (kernel == RC_KERNEL_MULTI_TRSV_FWD) {
+                if (f32) launch_multi_fwd_only<float>(a, s); else launch_multi_fwd_only<double>(a, s);
+            } else {
+                if (f32) launch_multi_bwd_only<float>(a, s); else launch_multi_bwd_only<double>(a, s);
+            }
+            RC_HIP(hipEventRecord(e1, s));
+        }
+        RC_HIP(hipGetLastError());
+        RC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        RC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RC_HIP(hipMemcpy(&sc, h->m_sc.p, sizeof(sc), hipMemcpyDeviceToHost));
+    h->m_solved = false;             // the timing overwrote the solve's vectors
+    if (sc.status & kPcgTimeout) return fail(h, RC_ERR_TIMEOUT, "timed kernel wait expired");
+    *ms_per_launch = total / reps;
+    if (bytes_per_launch) {
+        const double n = (double)h->ns, k = (double)h->m_k, sz = h->prec == 32 ? 4.0 : 8.0;
+        if (kernel == RC_KERNEL_MULTI_SPMM) *bytes_per_launch = 12.0 * (double)h->nnz_sys + 4.0 * (n + 1) + 16.0 * k * n;
+        else *bytes_per_launch = (4.0 + sz) * (double)h->nnz_lt + 4.0 * (n + 1) + 2.0 * sz * k * n;
+    }
+    return RC_OK;
+}
+
 static int rc_time_kernel_impl(rc_handle_t h, int kernel, int reps, double *ms_per_launch, double *bytes_per_launch)
 {
     RC_CHECK_HANDLE();
     if (h->host_only) return fail(h, RC_ERR_NO_DEVICE, "host-only handle");
+    if (kernel == RC_KERNEL_MULTI_SPMM || kernel == RC_KERNEL_MULTI_TRSV_FWD || kernel == RC_KERNEL_MULTI_TRSV_BWD) {
+        if (reps <= 0 || !ms_per_launch) return fail(h, RC_ERR_ARG, "bad timing arguments");
+        RC_HIP(hipSetDevice(h->opt.device));
+        return time_multi_kernel(h, kernel, reps, ms_per_launch, bytes_per_launch);
+    }
     if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and solve once before timing");
     {
         const int rc_alloc = alloc_solve(h);     // sizes the vectors, zeroes their dummy slot
@@ -1982,7 +2334,7 @@ static int rc_time_kernel_impl(rc_handle_t h, int kernel, int reps, double *ms_p
     sc.rz = 1.0;
     sc.rz0 = 1.0;
     RC_HIP(hipMemcpyAsync(h->scal.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
-    const double n = (double)h->ns, nnzA = (double)h->sys.nnz(), nnzL = (double)h->nnz_lt;
+    const double n = (double)h->ns, nnzA = (double)h->nnz_sys, nnzL = (double)h->nnz_lt;
     const double sz = h->prec == 32 ? 4.0 : 8.0;
     const double b_spmv = 12.0 * nnzA + 4.0 * (n + 1) + 16.0 * n;
     const double b_tri = (4.0 + sz) * nnzL + 4.0 * (n + 1) + 2.0 * sz * n;
@@ -2311,6 +2663,27 @@ int rc_autotune(rc_handle_t h, int reps, rc_tune_stats *st)
 int rc_get_solution(rc_handle_t h, double *x)
 {
     return guarded(h, [&]() { return rc_get_solution_impl(h, x); });
+}
+
+int rc_factor_ic0(rc_handle_t h, rc_factor_stats *st)
+{
+    return guarded(h, [&]() { return rc_factor_ic0_impl(h, st); });
+}
+
+int rc_set_solve_options(rc_handle_t h, int stop_rule, int maxit)
+{
+    return guarded(h, [&]() { return rc_set_solve_options_impl(h, stop_rule, maxit); });
+}
+
+int rc_get_multi_stats(rc_handle_t h, rc_solve_stats *st)
+{
+    return guarded(h, [&]() { return rc_get_multi_stats_impl(h, st); });
+}
+
+int rc_solve_group(int count, const rc_handle_t *handles, double tol, rc_solve_stats *st)
+{
+    rc_handle_t h0 = (count > 0 && handles) ? handles[0] : nullptr;
+    return guarded(h0, [&]() { return rc_solve_group_impl(count, handles, tol, st); });
 }
 
 }  // extern "C"

@@ -81,6 +81,16 @@ struct rc_handle_s {
     int nf = 0, ns = 0;
     int64_t nnz_low = 0;
 
+    // the loaded matrix on the device (b_*) and the device preparation's
+    // buffers: permutation and inverse, factored matrix when it differs from
+    // the PCG system (f_*), ground-append scratch (p_*)
+    DevVec b_ptr, b_ind, b_val, pfull, pinv, f_ptr, f_ind, f_val, p_cs, p_app, p_flag, p_pos;
+    bool base_dev = false;
+    bool fac_in_f = false;          // the factored matrix is f_* (ground appended or kept), else a_*
+    DevVec ic_up_ptr, ic_up_ind, ic_up_val, ic_l, ic_diag;   // IC(0) working set
+    int64_t nnz_sys = 0;            // nnz of the PCG system
+    double prepare_ms = 0.0;        // last preparation (ordering applied, lower slice, system)
+
     // factor inputs / working set
     DevVec lo_ptr, lo_row, lo_w, dep_init;
     DevVec big_list, big_count;
@@ -123,8 +133,13 @@ struct rc_handle_s {
     int sweep_blocks_max = 1024;    // occupancy limit of the persistent sweep grid
     bool rhs_set = false;
     std::vector<double> xref;
-    hipGraphExec_t graph = nullptr;
-    int graph_prec = 0;
+    int rhs_mode = 0;               // last resident right-hand side: 1 rc_set_rhs, 2 rc_set_rhs_multi
+    // captured iteration pairs: graph (single / batch run), m_graph (multi-RHS
+    // run); each is rebuilt when the buffers it was captured with change
+    hipGraphExec_t graph = nullptr, m_graph = nullptr;
+    const void *graph_key[3] = {nullptr, nullptr, nullptr};
+    int graph_prec_of[3] = {0, 0, 0};
+    int graph_K[3] = {0, 0, 0};
 
     // ordering network: graph + activations of the loaded system
     bool gnn_ready = false, gnn_keep = false;
@@ -149,11 +164,14 @@ struct rc_handle_s {
     DevVec m_x, m_r, m_ap, m_p0, m_p1, m_y, m_z, m_b, m_B, m_X, m_sc, m_part;
     int m_k = 0;                    // right-hand sides held in m_B (user order, column-major)
     int m_n = 0;                    // rows of every right-hand side in m_B
-    bool m_solved = false;
+    bool m_solved = false;          // m_x holds the solutions of the last multi-RHS run
+    bool m_ran = false;             // a multi-RHS run sized the vectors for this factor (kernel timing)
+    std::vector<rc_solve_stats> m_stats;   // per right-hand side of the last multi-RHS run
 
     ~rc_handle_s()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (m_graph) (void)hipGraphExecDestroy(m_graph);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -163,6 +181,15 @@ struct rc_handle_s {
         hipError_t e_ = (expr);                                                                \
         if (e_ != hipSuccess) {                                                                \
             h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                        \
+            return e_ == hipErrorOutOfMemory ? RC_ERR_NOMEM : RC_ERR_HIP;                      \
+        }                                                                                      \
+    } while (0)
+
+#define RC_HIP_H(hh, expr)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            (hh)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
             return e_ == hipErrorOutOfMemory ? RC_ERR_NOMEM : RC_ERR_HIP;                      \
         }                                                                                      \
     } while (0)
@@ -181,7 +208,9 @@ inline int fail(rc_handle_t h, int code, const std::string &msg)
 inline void invalidate_graph(rc_handle_t h)
 {
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    if (h->m_graph) (void)hipGraphExecDestroy(h->m_graph);
     h->graph = nullptr;
+    h->m_graph = nullptr;
 }
 
 // No C++ exception crosses the C ABI: host allocation failures become
@@ -211,6 +240,10 @@ int guarded(rc_handle_t h, F &&f) noexcept
     }
     return code;
 }
+
+// Device preparation (prep_device.cpp).
+int prepare_device(rc_handle_t h);
+int learned_order_device(rc_handle_t h, const float *scores_dev, int n, uint64_t seed, std::vector<int> &perm);
 
 // Shared between the host sources of the ABI (capi.cpp).
 SolveArgs solve_args(rc_handle_t h);

@@ -1,0 +1,217 @@
+"""On an MI355X through the C ABI: the IC(0) preconditioner against the
+oracle's, the device-side input preparation on every input kind, and the
+randomized factor and PCG against the oracle at the benchmark's own sizes
+(128^3 SDDM with ground, 128^3 log-uniform grid with an FP32 factor), plus the
+FP32-factor solution's error against an FP64 solve computed on the CPU."""
+import numpy as np
+import pytest
+
+from conftest import golden_path
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _body(rp, ci, va):
+    """Off-diagonal part of the CSR of L^T that rc_get_factor returns (unit diagonal first)."""
+    mask = np.ones(len(ci), bool)
+    mask[rp[:-1]] = False
+    return ci[mask], va[mask]
+
+
+# ------------------------------------------------------------------ IC(0)
+@pytest.mark.parametrize("name", ["grid3d_12_log", "rgg_2500", "sddm_10", "grid2d_30"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_ic0_factor_equals_oracle_and_preconditions(rc, orc, wl, name, prec):
+    say(f"[gpu] ic0 {name} prec={prec}")
+    if name == "sddm_10":
+        n, r, c, v = wl.grid_sddm((10, 10, 10))
+        kind = "sddm"
+    elif name == "rgg_2500":
+        n, r, c, v = wl.random_geometric_graph(2500, seed=7)
+        kind = "laplacian"
+    elif name == "grid2d_30":
+        n, r, c, v = wl.grid_laplacian((30, 30), "uniform_0.5_2", seed=3)
+        kind = "laplacian"
+    else:
+        n, r, c, v = wl.grid_laplacian((12, 12, 12), "loguniform", seed=7)
+        kind = "laplacian"
+    s = rc.Solver(device=0, precision=prec)
+    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if kind == "sddm" else rc.INPUT_LAPLACIAN)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    st = s.factor_ic0()
+    fac, sys_, perm, appended = orc.prepare_system(n, r, c, v, kind=kind, order="nnz-sort", order_seed=1)
+    f = orc.ic0(fac)
+    if appended:
+        f = orc.factor_trim_last(f)
+    rp, ci, va, dg = s.get_factor()
+    ind, val = _body(rp, ci, va)
+    np.testing.assert_array_equal(ind, f.ind)
+    if prec == 64:
+        np.testing.assert_array_equal(val, f.val)
+        np.testing.assert_array_equal(dg, f.diag)
+    else:
+        np.testing.assert_array_equal(val, f.val.astype(np.float32).astype(np.float64))
+        np.testing.assert_array_equal(dg, f.diag.astype(np.float32).astype(np.float64))
+    assert st["nnz_factor"] == st["nnz_lower"] + st["n"]       # IC(0): L keeps the lower pattern
+    b = s.make_rhs(rc.RHS_NORMAL if kind == "sddm" else rc.RHS_NORMAL_PROJECTED, 42)
+    x, so = s.solve(b, tol=1e-6)
+    xo, oo = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert abs(so["iterations"] - oo["iterations"]) <= 1, (so, oo)
+    assert np.linalg.norm(x[perm] - xo) <= (1e-7 if prec == 64 else 1e-4) * np.linalg.norm(xo)
+    s.close()
+
+
+def test_ic0_then_randomized_factor_on_one_handle(rc, orc, wl):
+    n, r, c, v = wl.grid_laplacian((16, 16, 16), "loguniform", seed=2)
+    s = rc.Solver(device=0, seed=5)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    s.factor_ic0()
+    s.factor()                                   # the randomized factor replaces it
+    fac, _, _, _ = orc.prepare_system(n, r, c, v, order="nnz-sort", order_seed=1)
+    f = orc.factor(fac, seed=5)
+    rp, ci, va, dg = s.get_factor()
+    ind, val = _body(rp, ci, va)
+    np.testing.assert_array_equal(ind, f.ind)
+    np.testing.assert_array_equal(val, f.val)
+    s.close()
+
+
+# ------------------------------------------------------------------ device preparation
+@pytest.mark.parametrize("order", ["nnz-sort", "random", "nd"])
+def test_device_preparation_gives_the_oracle_factor(rc, orc, wl, order):
+    """The prepared system (permutation, ground append, lower slice, dependency
+    counts) is built on the device: the factor still equals the oracle's."""
+    say(f"[gpu] device prep order={order}")
+    n, r, c, v = wl.grid_sddm((9, 8, 7))
+    s = rc.Solver(device=0, seed=3)
+    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM)
+    meth = {"nnz-sort": rc.ORDER_NNZ_SORT, "random": rc.ORDER_RANDOM, "nd": rc.ORDER_ND}[order]
+    s.order(meth, 4)
+    st = s.factor()
+    assert st["prepare_ms"] > 0
+    assert s.factor()["prepare_ms"] == 0.0                 # reused
+    p = s.permutation()
+    a = orc.laplacian(n, r, c, v)
+    g, ok = orc.ground_append(orc.permute(a, p))
+    assert ok
+    f = orc.factor_trim_last(orc.factor(g, seed=3))
+    rp, ci, va, dg = s.get_factor()
+    ind, val = _body(rp, ci, va)
+    np.testing.assert_array_equal(ind, f.ind)
+    np.testing.assert_array_equal(val, f.val)
+    np.testing.assert_array_equal(dg, f.diag)
+    s.close()
+
+
+def test_device_preparation_of_a_grounded_file_and_rejections(rc, orc, wl, tmp_path):
+    # the same SDDM written with its ground already appended (driver_physics.cu input)
+    n, r, c, v = wl.grid_sddm((6, 6, 6))
+    h = rc.Solver(device=-1)
+    h.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM)
+    h.order(rc.ORDER_AS_IS, 1)
+    path = str(tmp_path / "g.mtx")
+    h.write_prepared_mtx(path)
+    s = rc.Solver(device=0, seed=2)
+    s.load_mtx(path, kind=rc.INPUT_GROUNDED)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    s.factor()
+    t = rc.Solver(device=-1)
+    t.load_mtx(path, kind=rc.INPUT_GROUNDED)
+    t.order(rc.ORDER_NNZ_SORT, 1)
+    nn, ptr, ind, val = t.prepared_csc()
+    f = orc.factor_trim_last(orc.factor(orc.Csc(nn, ptr, ind, val), seed=2))
+    rp, ci, va, dg = s.get_factor()
+    bi, bv = _body(rp, ci, va)
+    np.testing.assert_array_equal(bi, f.ind)
+    np.testing.assert_array_equal(bv, f.val)
+    # a non-dominant SDDM is refused by the device preparation too
+    bad = rc.Solver(device=0)
+    bad.load_mtx(golden_path("nondominant.mtx"), kind=rc.INPUT_SDDM)
+    with pytest.raises(rc.RcholError) as e:
+        bad.factor()
+    assert e.value.status == rc.ERR_NOT_DOMINANT
+    for x in (s, bad):
+        x.close()
+
+
+# ------------------------------------------------------------------ benchmark sizes
+def _full_size_case(rc, orc, wl, which):
+    if which == "sddm128":
+        n, r, c, v = wl.grid_sddm((128, 128, 128), 1e-2)
+        kind, prec, rhs = "sddm", 64, rc.RHS_NORMAL
+    else:
+        n, r, c, v = wl.grid_laplacian((128, 128, 128), "loguniform", seed=7)
+        kind, prec, rhs = "laplacian", 32, rc.RHS_NORMAL_PROJECTED
+    return (n, r, c, v), kind, prec, rhs
+
+
+@pytest.mark.parametrize("which", ["sddm128", "grid128_log_f32"])
+def test_benchmark_size_factor_and_pcg_equal_oracle(rc, orc, wl, which):
+    """128^3 (2.1 M rows): the 128^3 SDDM workload as benchmarked (ground
+    appended, nnz-sort, FP64) and the log-uniform grid generator of the 256^3
+    workload at 128^3 with an FP32 factor (max degree ~500: the large-LDS
+    elimination path and the autotuned sweeps)."""
+    say(f"[gpu] benchmark size {which}")
+    (n, r, c, v), kind, prec, rhs = _full_size_case(rc, orc, wl, which)
+    s = rc.Solver(device=0, seed=1, precision=prec)
+    s.set_triplets(n, r, c, v, kind=rc.INPUT_SDDM if kind == "sddm" else rc.INPUT_LAPLACIAN)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    st = s.factor()
+    b = s.make_rhs(rhs, 42)
+    s.set_rhs(b)
+    s.autotune(reps=1)
+    x, so = s.solve(b, tol=1e-6)
+    rp, ci, va, dg = s.get_factor()
+    fac, sys_, perm, appended = orc.prepare_system(n, r, c, v, kind=kind, order="nnz-sort", order_seed=1)
+    f = orc.factor_omp(fac, seed=1, threads=0)
+    assert st["nnz_factor"] == f.stats["nnz_off"] + fac.n
+    assert st["nlevels"] > 0
+    if appended:
+        f = orc.factor_trim_last(f)
+    ind, val = _body(rp, ci, va)
+    np.testing.assert_array_equal(ind, f.ind)
+    if prec == 64:
+        np.testing.assert_array_equal(val, f.val)
+        np.testing.assert_array_equal(dg, f.diag)
+    else:
+        np.testing.assert_array_equal(val, f.val.astype(np.float32).astype(np.float64))
+    del rp, ci, va
+    xo, oo = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert abs(so["iterations"] - oo["iterations"]) <= 1, (so, oo)
+    assert so["converged"] == 1 and oo["converged"] == 1
+    # A-norm agreement (the FP32 sweeps round differently from the oracle's
+    # sequential solves; PCG minimises the A-norm error)
+    d = x[perm] - xo
+    assert np.sqrt(d @ orc.spmv(sys_, d)) <= (1e-8 if prec == 64 else 1e-5) * np.sqrt(xo @ orc.spmv(sys_, xo))
+    assert abs(so["rel_res_true"] - oo["rel_res_true"]) <= (0.05 if prec == 64 else 0.25) * oo["rel_res_true"]
+    s.close()
+
+
+def test_fp32_factor_error_against_cpu_fp64_solve(rc, orc, wl):
+    """The FP32-factor solution of the log-uniform grid (64^3) against an FP64
+    reference computed on the CPU only (the oracle's FP64 factor, its PCG to
+    1e-12): the A-norm error sits at the stopping tolerance."""
+    say("[gpu] fp32 error vs CPU fp64")
+    n, r, c, v = wl.grid_laplacian((64, 64, 64), "loguniform", seed=7)
+    s = rc.Solver(device=0, seed=1, precision=32)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    fac, sys_, perm, _ = orc.prepare_system(n, r, c, v, order="nnz-sort", order_seed=1)
+    f = orc.factor_omp(fac, seed=1)
+    xr, so = orc.pcg_omp(sys_, f, b[perm], tol=1e-12, maxit=3000)
+    assert so["converged"] == 1 and so["rel_res_true"] < 1e-10
+    x_ref = np.empty(n)
+    x_ref[perm] = xr
+    s.set_reference_solution(x_ref)
+    x, st = s.solve(b, tol=1e-6)
+    err = st["err_vs_oracle"]
+    say(f"[gpu] fp32-factor A-norm error {err:.3e}, {st['iterations']} iterations")
+    assert 0 <= err <= 1e-5
+    s.close()
