@@ -18,7 +18,8 @@ Both are also run with the true-residual stop rule (||b - A x|| / ||b|| <=
 1e-6) so the speed-up is stated at equal accuracy too, and the FP32-factor
 solution's error against an FP64 reference solve is reported.
 
-Other workloads (--config): c1, c2, c3, c3_f64, c3_learned, c3_nd, c4 (64
+Other workloads (--config): c1, c2, c3, c3_f64, c3_learned, c3_nd, c3_ic0 / c2_ic0 (IC(0)-PCG, the
+context preconditioner, on the same systems), c4 (64
 right-hand sides on a 2048^2 grid and 200 random geometric graphs in one
 launch sequence), c4a, c4b.
 
@@ -70,6 +71,9 @@ CONFIGS = {
     "c3_f64": dict(C3, workload="grid3d_256_loguniform_nnzsort_f64", prec=64),
     "c3_learned": dict(C3, workload="grid3d_256_loguniform_learned_f32factor", prec=32, order="learned"),
     "c3_nd": dict(C3, workload="grid3d_256_loguniform_nd_f32factor", prec=32, order="nd"),
+    "c3_ic0": dict(C3, workload="grid3d_256_loguniform_nnzsort_ic0_f64", prec=64, factor="ic0"),
+    "c2_ic0": dict(workload="sddm3d_128_ground_nnzsort_ic0", dims=(128, 128, 128), kind="sddm", prec=64, rhs="normal",
+                   factor="ic0"),
     "c4a": dict(workload="grid2d_2048_uniform_0.5_2_nnzsort_f32factor_64rhs", dims=(2048, 2048), kind="laplacian",
                 weights="uniform_0.5_2", prec=32, rhs="normal_projected", nrhs=64, wseed=11),
     "c4b": dict(workload="rgg_batch_200_n10k-200k_nnzsort_f32factor", kind="rgg_batch", nsys=200, prec=32,
@@ -170,12 +174,15 @@ class Run:
             return self.summarize(self.s.solve_multi_resident(tol=self.args.tol))
         return self.s.solve_resident(self.args.tol)
 
+    def factor(self):
+        return self.s.factor_ic0() if self.cfg.get("factor") == "ic0" else self.s.factor()
+
     def warmup(self, steps):
-        self.fst = self.s.factor()
+        self.fst = self.factor()
         self.set_rhs()
         for w in range(max(steps, 1)):
             t = time.perf_counter()
-            self.fst = self.s.factor()
+            self.fst = self.factor()
             sst = self.solve()
             log(f"[bench] {self.name} warmup {w}: {time.perf_counter() - t:.3f}s iters={sst['iterations']} "
                 f"factor={self.fst['factor_ms']:.1f}ms assemble={self.fst['assemble_ms']:.1f}ms pcg={sst['pcg_ms']:.1f}ms")
@@ -197,7 +204,7 @@ class Run:
         t_all = time.perf_counter()
         for k in range(steps):
             t = time.perf_counter()
-            fst = self.s.factor()
+            fst = self.factor()
             sst = self.solve()
             times.append(time.perf_counter() - t)
             fms.append(fst["factor_ms"]); ams.append(fst["assemble_ms"]); pms.append(sst["pcg_ms"])
@@ -344,6 +351,17 @@ def cpu_baseline(wl, cfg, args, dims=None, nrhs=1):
     return out
 
 
+def component_means(n, rows, cols, b):
+    """Mean of b over each vertex's connected component (RHS projection)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    g = sp.coo_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+    nc, lab = connected_components(g, directed=False)
+    s = np.bincount(lab, weights=b, minlength=nc)
+    c = np.bincount(lab, minlength=nc)
+    return (s / c)[lab]
+
+
 def cpu_baseline_batch(wl, cfg, args, max_n):
     """The CPU restatement on the first systems of the batch, each solved
     alone on all host threads, until max_n rows are covered; time scaled by
@@ -362,7 +380,7 @@ def cpu_baseline_batch(wl, cfg, args, max_n):
         n, r, c, v = wl.random_geometric_graph(sizes[i], seed=1000 + i)
         fac, sys_, perm, _ = orc.prepare_system(n, r, c, v, kind="laplacian", order="nnz-sort", order_seed=1)
         b = orc.normal_vector(42, n)
-        b -= b.mean()
+        b -= component_means(n, r, c, b)          # consistent per connected component, as on the GPU
         t0 = time.perf_counter()
         f = orc.factor_omp(fac, seed=1, threads=threads)
         t1 = time.perf_counter()

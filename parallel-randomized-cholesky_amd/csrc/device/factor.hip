@@ -305,7 +305,7 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
 // Steps 2-5 of a column whose m >= 1 entries are gathered in b.  Inlined at
 // every call so that b's accesses compile to LDS or global instructions (a
 // generic pointer would make each LDS access wait for all memory traffic).
-template <int kMaxR>
+template <int kMaxR, bool kAsync>
 __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int round, const ColBuf &b, int m,
                                                Cursor &fcur, Cursor &ocur, int *next, unsigned int *next_set)
 {
@@ -401,12 +401,66 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
     // a column is eliminated in the round after its last dependency, so its
     // level in the triangular-solve DAG is its round (:638-645)
     if (lane == 0) {
-        a.level[k] = round;
+        if (!kAsync) a.level[k] = round;     // the queue-driven tail's levels come from atomicMax
         a.out_start[k] = obase;
         a.out_cnt[k] = d;
         a.diag[k] = S;
     }
     __syncthreads();   // minc and jt complete
+
+    if constexpr (kAsync) {
+        // ---- 5 (queue-driven tail). A neighbour may be taken by another wave
+        //      as soon as its dependency count reaches zero, so every fill push,
+        //      factor entry and level of this column is made visible (release
+        //      fence) before any dependency update; ready neighbours go to the
+        //      queue after an acquire-release fence (the other columns' pushes
+        //      into them are then visible to whoever takes them).
+        const int lk = round + 1;
+        for (int base_i = 0; base_i < d; base_i += kWave) {
+            const int e = base_i + lane;
+            if (e < d) {
+                const int r = b.mrow[e];
+                a.out_row[obase + e] = r;
+                a.out_val[obase + e] = -(b.mw[e] / S);
+                if (r != last) atomicMax(&a.level[r], lk);
+            }
+            if (e < nfill_out) {
+                const int j = b.jt[e];
+                const int ri = b.srow[e], rj = b.srow[j];
+                const int rr = ri > rj ? ri : rj;
+                const int cc = ri > rj ? rj : ri;
+                const double rest = S - b.C[e];
+                const double fw = (b.sw[e] * rest) / S;
+                const int rec = (int)(fbase + e);
+                FillRec fr;
+                fr.row = rr;
+                fr.w = fw;
+                fr.next = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
+                a.pool[rec] = fr;
+            }
+        }
+        fence_release();
+        for (int base_i = 0; base_i < d; base_i += kWave) {
+            const int e = base_i + lane;
+            bool ready = false;
+            int r = -1;
+            if (e < d) {
+                r = b.mrow[e];
+                const int delta = b.minc[e] - b.mmult[e];
+                if (r != last && delta != 0) ready = atomicAdd(&a.dep[r], delta) + delta == 0;
+            }
+            fence_acq_rel();
+            const uint64_t mask = ballot(ready);
+            if (mask) {
+                unsigned int qb = 0;
+                if (lane == 0) qb = atomicAdd(a.aq_ctr + kAqTail, (unsigned int)__popcll(mask));
+                qb = bcast(qb, 0);
+                if (ready) st_relaxed(&next[qb + lanes_below(mask)], r);
+            }
+        }
+        __syncthreads();
+        return true;
+    }
 
     // ---- 5. per 64 entries: the neighbour's net dependency update (:653-687),
     //         the fill edge (max, min, w_i (S - C_i) / S) pushed onto column
@@ -450,7 +504,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
 // kDefer (round kernel): a column larger than the LDS buffer is appended to
 // the round's deferred list; otherwise (large-LDS kernel) it is gathered
 // again into this workgroup's global scratch.
-template <int kMaxR, bool kDefer>
+template <int kMaxR, bool kDefer, bool kAsync = false>
 __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *lds, int lds_cap, Cursor &fcur,
                                  Cursor &ocur, int *next, unsigned int *next_set, unsigned int *defer_count)
 {
@@ -468,7 +522,7 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
     int m = gather_column<kDefer>(a, lo0, norig, f0, b, lds_cap);
     if (m == 0) {                      // empty column: zero pivot (driver_local.cpp:491-498)
         if (lane == 0) {
-            a.level[k] = round;
+            if (!kAsync) a.level[k] = round;
             a.out_start[k] = 0;
             a.out_cnt[k] = 0;
             a.diag[k] = 0.0;
@@ -494,10 +548,10 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
                 if (lane == 0) raise_status(a.ctr, kStatusInconsistent);
                 return false;
             }
-            return process_column<kMaxR>(a, k, round, g, m, fcur, ocur, next, next_set);
+            return process_column<kMaxR, kAsync>(a, k, round, g, m, fcur, ocur, next, next_set);
         }
     }
-    return process_column<kMaxR>(a, k, round, b, m, fcur, ocur, next, next_set);
+    return process_column<kMaxR, kAsync>(a, k, round, b, m, fcur, ocur, next, next_set);
 }
 
 }  // namespace
@@ -569,6 +623,86 @@ __global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int ro
     }
 }
 
+// ---- queue-driven tail -----------------------------------------------------
+// Once a round holds at most kBigGrid columns, the rest of the elimination runs
+// in this one persistent launch: every wave takes the next queue index, waits
+// until that slot holds a column (slots start at -1), eliminates it in the
+// large LDS buffer and appends the neighbours it made ready.  A column's level
+// is 1 + the largest level of its lower neighbours (atomicMax by each of them
+// before its dependency update), the same value the rounds assign.  Slot h is
+// written by the wave that made its column ready, which is running, so every
+// wait ends; the waits are bounded all the same (status bit on expiry).
+__device__ __forceinline__ bool factor_should_stop(const FactorArgs &a)
+{
+    return ld_relaxed(&a.ctr[kCtrStatus]) != 0ull;
+}
+
+__global__ void __launch_bounds__(kWave) k_factor_async(FactorArgs a, unsigned int total)
+{
+    extern __shared__ __attribute__((aligned(16))) char lds_big[];
+    const int lane = lane_id();
+    Cursor fcur, ocur;
+    long long *ws = a.wave_state + 4 * ((size_t)a.big_ws + blockIdx.x);
+    fcur.cur = ws[0]; fcur.end = ws[1];
+    ocur.cur = ws[2]; ocur.end = ws[3];
+    while (true) {
+        unsigned int h = 0;
+        if (lane == 0) h = atomicAdd(a.aq_ctr + kAqHead, 1u);
+        h = bcast(h, 0);
+        if (h >= total) break;
+        int k = -1;
+        if (lane == 0) {
+            k = ld_relaxed(&a.aq[h]);
+            if (k < 0) {
+                const long long t0 = wall_clock64();
+                int nap = 1;
+                while ((k = ld_relaxed(&a.aq[h])) < 0) {
+                    if (factor_should_stop(a)) break;
+                    if (wall_clock64() - t0 > a.spin_ticks) {
+                        raise_status(a.ctr, kStatusTimeout);
+                        break;
+                    }
+                    for (int t = 0; t < nap; ++t) __builtin_amdgcn_s_sleep(2);
+                    nap = nap < 8 ? nap * 2 : 8;
+                }
+            }
+        }
+        k = bcast(k, 0);
+        if (k < 0) break;                          // stopped (status raised)
+        fence_acquire();
+        const int lvl = a.level[k];
+        if (!eliminate_column<2, false, true>(a, k, lvl, lds_big, kBigCap, fcur, ocur, a.aq, nullptr, nullptr))
+            break;
+        if (factor_should_stop(a)) break;
+    }
+    if (lane == 0) {
+        ws[0] = fcur.cur; ws[1] = fcur.end;
+        ws[2] = ocur.cur; ws[3] = ocur.end;
+    }
+}
+
+// The queue starts with round `round`'s list (its columns' level is the round).
+__global__ void k_factor_async_init(FactorArgs a, int round, const int *cur, unsigned int total)
+{
+    unsigned int c, incl;
+    list_counts(a, list_set(a, round), c, incl);
+    const unsigned int ncur = __shfl(incl, kWave - 1, kWave);
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+        a.aq[i] = -1;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (unsigned int i = threadIdx.x; i < ncur; i += blockDim.x) {
+            const int col = cur[list_index(a, c, incl, i)];
+            a.aq[i] = col;
+            a.level[col] = round;
+        }
+        if (threadIdx.x == 0) {
+            a.aq_ctr[kAqHead] = 0u;
+            a.aq_ctr[kAqTail] = ncur;
+        }
+    }
+}
+
 // ---- set-up kernels ------------------------------------------------------
 
 __global__ void k_factor_init(int n, const int *dep_init, int *dep, int *head, int *level, int *out_cnt,
@@ -617,6 +751,16 @@ void factor_kernel_setup()
 {
     (void)hipFuncSetAttribute((const void *)k_factor_round_big, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)((size_t)kBigCap * kBytesPerEntry));
+    (void)hipFuncSetAttribute((const void *)k_factor_async, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)kBigCap * kBytesPerEntry));
+}
+
+void launch_factor_async(const FactorArgs &a, int round, unsigned int total, hipStream_t s)
+{
+    const int *cur = (round & 1) ? a.list1 : a.list0;
+    // one block: the init kernel's slot clearing must finish before it seeds the queue
+    hipLaunchKernelGGL(k_factor_async_init, dim3(1), dim3(1024), 0, s, a, round, cur, total);
+    hipLaunchKernelGGL(k_factor_async, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, a, total);
 }
 
 int factor_lds_capacity() { return kLdsCap; }

@@ -30,7 +30,11 @@ enum : unsigned long long {
     kStatusOutPool = 2,
     kStatusScratch = 4,
     kStatusInconsistent = 16,
+    kStatusTimeout = 32,
 };
+
+// Counters of the queue-driven tail (FactorArgs::aq_ctr), a cache line apart.
+constexpr int kAqHead = 0, kAqTail = 32, kAqWords = 64;
 
 // Pending fills of a column are kept in kFillLists singly linked lists
 // (head[k * kFillLists + h]; a record goes to list (record index mod
@@ -81,11 +85,19 @@ struct FactorArgs {
     char *scratch;          // overflow buffers of the kBigGrid deferred-column workgroups
     int scratch_cap;        // entries per workgroup (power of two)
     uint64_t seed;
+    // queue-driven tail: column queue (n entries, -1 = not yet written) and
+    // its head / tail counters; bounded waits
+    int *aq;
+    unsigned int *aq_ctr;
+    long long spin_ticks;
 };
 
 void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s);
 void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s);
 void launch_factor_finish(const FactorArgs &a, hipStream_t s);
+// The rest of the elimination from round `round` (whose list holds at most
+// kBigGrid columns) in one persistent launch; `total` columns remain.
+void launch_factor_async(const FactorArgs &a, int round, unsigned int total, hipStream_t s);
 int factor_lds_capacity();
 int factor_scratch_bytes_per_entry();
 long long factor_pool_slack(int num_blocks);

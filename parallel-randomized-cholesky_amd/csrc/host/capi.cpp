@@ -114,6 +114,8 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
         const char *sc = std::getenv("RCHOL_LIST_SEG_CAP");     // ready-list segment capacity limit (>= 1)
         if (sc) h->seg_cap_limit = std::max(1, std::atoi(sc));
+        const char *at = std::getenv("RCHOL_ASYNC_TAIL");       // 0: narrow rounds stay round kernels
+        if (at) h->async_tail = std::atoi(at) != 0;
     }
     *out = h;
     return RC_OK;
@@ -592,6 +594,8 @@ static int alloc_factor_work(rc_handle_t h)
     RC_HIP(h->out_val.ensure((size_t)h->out_cap * sizeof(double)));
     // only the deferred-column kernel (kBigGrid workgroups) uses the scratch
     RC_HIP(h->scratch.ensure((size_t)kBigGrid * (size_t)h->scratch_cap * (size_t)factor_scratch_bytes_per_entry()));
+    RC_HIP(h->aq.ensure(nf * sizeof(int)));
+    RC_HIP(h->aq_ctr.ensure(kAqWords * sizeof(unsigned int)));
     return RC_OK;
 }
 
@@ -627,6 +631,9 @@ static FactorArgs factor_args(rc_handle_t h)
     a.scratch = h->scratch.as<char>();
     a.scratch_cap = h->scratch_cap;
     a.seed = h->opt.seed;
+    a.aq = h->aq.as<int>();
+    a.aq_ctr = h->aq_ctr.as<unsigned int>();
+    a.spin_ticks = h->spin_ticks;
     return a;
 }
 
@@ -660,6 +667,8 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         // rounds in growing batches; a round whose list is empty returns at once
         int round = 0, batch = 8;
         unsigned long long st = 0;
+        unsigned long long async_total = 0;      // columns eliminated by the queue-driven tail
+        int async_round = -1;
         unsigned int *probe = nullptr;     // the next round's list counters, then the status word
         RC_HIP(hipHostMalloc((void **)&probe, (kListSlots + 2) * sizeof(unsigned int), hipHostMallocDefault));
         while (true) {
@@ -677,6 +686,20 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
             unsigned long long pending = probe[kListSegs];
             for (int q = 0; q < kListSegs; ++q) pending += std::min<unsigned long long>(probe[q], (unsigned long long)a.seg_cap);
             if (st != 0 || pending == 0) break;
+            if (h->async_tail && pending <= (unsigned long long)kBigGrid) {
+                // narrow from here: the rest runs in one queue-driven launch
+                std::vector<unsigned int> cnt((size_t)round);
+                e = hipMemcpyAsync(cnt.data(), h->round_count.p, cnt.size() * sizeof(unsigned int),
+                                   hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) { (void)hipHostFree(probe); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
+                unsigned long long before = 0;
+                for (unsigned int v : cnt) before += v;
+                async_total = (unsigned long long)(h->nf - 1) - before;
+                async_round = round;
+                launch_factor_async(a, round, (unsigned int)async_total, s);
+                break;
+            }
             batch = std::min(batch * 2, 64);     // bounds the empty rounds launched past the end
         }
         (void)hipHostFree(probe);
@@ -696,9 +719,10 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         *ms_out = ms;
         st = c[kCtrStatus];
         if (st == 0) {
-            unsigned long long done = 0;
+            unsigned long long done = async_total;
             int used = 0;
-            for (size_t i = 0; i < counts.size(); ++i) {
+            const size_t nc = async_round >= 0 ? (size_t)async_round : counts.size();
+            for (size_t i = 0; i < nc; ++i) {
                 done += counts[i];
                 if (counts[i]) used = (int)i + 1;
             }
@@ -716,6 +740,7 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         }
         if (st & kStatusInconsistent)
             return fail(h, RC_ERR_STATE, "elimination kernel found an inconsistent fill list");
+        if (st & kStatusTimeout) return fail(h, RC_ERR_TIMEOUT, "elimination queue wait timed out");
         if (st & kStatusFillPool) {
             if (h->pool_cap >= kMaxPoolRecords)
                 return fail(h, RC_ERR_POOL, "fill pool would exceed 2^31 - 1 records (int32 record indices)");

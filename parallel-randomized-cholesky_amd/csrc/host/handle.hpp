@@ -96,6 +96,8 @@ struct rc_handle_s {
     DevVec big_list, big_count;
     DevVec dep, head, level, list0, list1, seg_count, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
         out_cnt, diag64, scratch;
+    DevVec aq, aq_ctr;              // queue of the elimination's queue-driven tail
+    bool async_tail = true;         // narrow rounds run as one queue-driven launch
     long long pool_cap = 0, out_cap = 0;
     int scratch_cap = 0;
     int rounds = 0;
