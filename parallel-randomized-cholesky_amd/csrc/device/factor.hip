@@ -455,7 +455,11 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
                 unsigned int qb = 0;
                 if (lane == 0) qb = atomicAdd(a.aq_ctr + kAqTail, (unsigned int)__popcll(mask));
                 qb = bcast(qb, 0);
-                if (ready) st_relaxed(&next[qb + lanes_below(mask)], r);
+                // every remaining column is queued once: a slot past the
+                // queue's end means inconsistent dependency counts
+                const unsigned int slot = qb + (unsigned int)lanes_below(mask);
+                if (ready && slot < a.aq_total) st_relaxed(&next[slot], r);
+                if (ready && slot >= a.aq_total) raise_status(a.ctr, kStatusInconsistent);
             }
         }
         __syncthreads();
@@ -682,24 +686,31 @@ __global__ void __launch_bounds__(kWave) k_factor_async(FactorArgs a, unsigned i
 }
 
 // The queue starts with round `round`'s list (its columns' level is the round).
+// One workgroup.  Entry i of the list is found per thread from the slot
+// counts kept in LDS (list_index needs a wave-uniform entry).
 __global__ void k_factor_async_init(FactorArgs a, int round, const int *cur, unsigned int total)
 {
-    unsigned int c, incl;
-    list_counts(a, list_set(a, round), c, incl);
-    const unsigned int ncur = __shfl(incl, kWave - 1, kWave);
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
-        a.aq[i] = -1;
+    __shared__ unsigned int s_c[kWave], s_incl[kWave];
+    if (threadIdx.x < kWave) {
+        unsigned int c, incl;
+        list_counts(a, list_set(a, round), c, incl);
+        s_c[threadIdx.x] = c;
+        s_incl[threadIdx.x] = incl;
+    }
+    for (unsigned int i = threadIdx.x; i < total; i += blockDim.x) a.aq[i] = -1;
     __syncthreads();
-    if (blockIdx.x == 0) {
-        for (unsigned int i = threadIdx.x; i < ncur; i += blockDim.x) {
-            const int col = cur[list_index(a, c, incl, i)];
-            a.aq[i] = col;
-            a.level[col] = round;
-        }
-        if (threadIdx.x == 0) {
-            a.aq_ctr[kAqHead] = 0u;
-            a.aq_ctr[kAqTail] = ncur;
-        }
+    const unsigned int ncur = min(s_incl[kWave - 1], total);
+    for (unsigned int i = threadIdx.x; i < ncur; i += blockDim.x) {
+        int s = 0;                                   // slot of entry i: the first with incl > i
+        while (s < kWave - 1 && s_incl[s] <= i) ++s;
+        const size_t idx = (size_t)s * (size_t)a.seg_cap + (i - (s_incl[s] - s_c[s]));
+        const int col = cur[idx];
+        a.aq[i] = col;
+        a.level[col] = round;
+    }
+    if (threadIdx.x == 0) {
+        a.aq_ctr[kAqHead] = 0u;
+        a.aq_ctr[kAqTail] = ncur;
     }
 }
 
@@ -755,8 +766,10 @@ void factor_kernel_setup()
                               (int)((size_t)kBigCap * kBytesPerEntry));
 }
 
-void launch_factor_async(const FactorArgs &a, int round, unsigned int total, hipStream_t s)
+void launch_factor_async(const FactorArgs &a0, int round, unsigned int total, hipStream_t s)
 {
+    FactorArgs a = a0;
+    a.aq_total = total;            // total <= n - 1 slots of the n-entry queue
     const int *cur = (round & 1) ? a.list1 : a.list0;
     // one block: the init kernel's slot clearing must finish before it seeds the queue
     hipLaunchKernelGGL(k_factor_async_init, dim3(1), dim3(1024), 0, s, a, round, cur, total);

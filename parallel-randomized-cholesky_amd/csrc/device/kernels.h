@@ -89,6 +89,7 @@ struct FactorArgs {
     // its head / tail counters; bounded waits
     int *aq;
     unsigned int *aq_ctr;
+    unsigned int aq_total;  // slots of the queue in use (set by launch_factor_async)
     long long spin_ticks;
 };
 

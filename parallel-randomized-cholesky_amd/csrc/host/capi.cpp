@@ -632,6 +632,7 @@ static FactorArgs factor_args(rc_handle_t h)
     a.scratch_cap = h->scratch_cap;
     a.seed = h->opt.seed;
     a.aq = h->aq.as<int>();
+    a.aq_total = 0;
     a.aq_ctr = h->aq_ctr.as<unsigned int>();
     a.spin_ticks = h->spin_ticks;
     return a;
@@ -711,6 +712,10 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         std::vector<unsigned int> counts((size_t)round + 1);
         RC_HIP(hipMemcpyAsync(counts.data(), h->round_count.p, counts.size() * sizeof(unsigned int),
                               hipMemcpyDeviceToHost, s));
+        unsigned int aq_tail = 0;
+        if (async_round >= 0)
+            RC_HIP(hipMemcpyAsync(&aq_tail, h->aq_ctr.as<unsigned int>() + kAqTail, sizeof(aq_tail),
+                                  hipMemcpyDeviceToHost, s));
         RC_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
@@ -726,6 +731,10 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
                 done += counts[i];
                 if (counts[i]) used = (int)i + 1;
             }
+            // the queue-driven tail queued every column it was given
+            if (async_round >= 0 && aq_tail != async_total)
+                return fail(h, RC_ERR_STATE, "queue-driven elimination queued " + std::to_string(aq_tail) + " of " +
+                                                 std::to_string(async_total) + " columns");
             if (done != (unsigned long long)(h->nf - 1))
                 return fail(h, RC_ERR_STATE, "elimination did not retire every column (" + std::to_string(done) + "/" +
                                                  std::to_string(h->nf - 1) + ")");

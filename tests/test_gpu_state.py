@@ -75,3 +75,37 @@ def test_multi_rhs_true_residual_rule_reports_unconverged(rc, wl):
     _, so = s.solve(B[:, 0], tol=1e-10)
     assert so["iterations"] == 3 and so["converged"] == 0
     s.close()
+
+
+@pytest.mark.parametrize("name", ["grid3d_24_log", "rgg_20000", "star_3000"])
+def test_queue_driven_tail_equals_round_kernels(rc, wl, monkeypatch, name):
+    """The narrow rounds eliminated by one queue-driven launch give the same
+    factor and the same level schedule as the round kernels."""
+    print(f"[gpu] queue-driven tail {name}", flush=True)
+    if name == "grid3d_24_log":
+        n, r, c, v = wl.grid_laplacian((24, 24, 24), "loguniform", seed=5)
+    elif name == "rgg_20000":
+        n, r, c, v = wl.random_geometric_graph(20000, seed=4)
+    else:
+        d = 3000                                    # one column above the large-LDS capacity
+        u = np.zeros(d, np.int64)
+        w = np.arange(1, d + 1)
+        n, r, c, v = wl.laplacian_triplets(d + 1, u, w, np.linspace(0.5, 3.0, d))
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("RCHOL_ASYNC_TAIL", mode)
+        s = rc.Solver(device=0, seed=9)
+        s.set_triplets(n, r, c, v)
+        s.order(rc.ORDER_AS_IS if name.startswith("star") else rc.ORDER_NNZ_SORT, 1)
+        st = s.factor()
+        b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 3)
+        x, so = s.solve(b, tol=1e-8)
+        out[mode] = (st, s.get_factor(), x, so)
+        s.close()
+    a, b_ = out["0"], out["1"]
+    for k in ("nnz_factor", "fills", "nlevels", "max_degree", "zero_pivots"):
+        assert a[0][k] == b_[0][k], k
+    for p, q in zip(a[1], b_[1]):
+        np.testing.assert_array_equal(p, q)
+    np.testing.assert_array_equal(a[2], b_[2])
+    assert a[3]["iterations"] == b_[3]["iterations"]
