@@ -246,9 +246,15 @@ __global__ void k_ell_width(int nslices, const int *start, const int *cnt, const
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * blockDim.x / kWave;
     for (int c = wave; c < nslices; c += nwaves) {
-        int row;
-        slice_row(c, start, cnt, order, nrows, lane, row);
-        int len = row >= 0 ? ptr[row + 1] - ptr[row] : 0;
+        // a chunk holds up to 256 rows: 64 at a time
+        const int rows = start ? cnt[c] : kWave;
+        int len = 0;
+        for (int i0 = 0; i0 < rows; i0 += kWave) {
+            int row;
+            slice_row(c, start, cnt, order, nrows, i0 + lane, row);
+            const int l = row >= 0 ? ptr[row + 1] - ptr[row] : 0;
+            len = l > len ? l : len;
+        }
         for (int off = 32; off > 0; off >>= 1) {
             int o = __shfl_xor(len, off, kWave);
             len = o > len ? o : len;
@@ -346,16 +352,16 @@ __global__ void k_level_width(int ns, const int *order, const int *level, const 
     }
 }
 
-__global__ void k_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt)
+__global__ void k_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int gmin, int *cnt)
 {
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlevels; l += gridDim.x * blockDim.x) {
-        const int r = chunk_rows(lw[l]);
+        const int r = chunk_rows(lw[l], gmin);
         cnt[l] = (lvl_ptr[l + 1] - lvl_ptr[l] + r - 1) / r;
     }
 }
 
 __global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
-                             int *chunk_start, int *chunk_cnt)
+                             int gmin, int *chunk_start, int *chunk_cnt)
 {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
         int lo = 0, hi = nlevels - 1;          // last level with chunk_ptr[l] <= c
@@ -363,7 +369,7 @@ __global__ void k_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const
             int mid = (lo + hi + 1) >> 1;
             if (chunk_ptr[mid] <= c) lo = mid; else hi = mid - 1;
         }
-        const int r = chunk_rows(lw[lo]);
+        const int r = chunk_rows(lw[lo], gmin);
         int start = lvl_ptr[lo] + (c - chunk_ptr[lo]) * r;
         int end = lvl_ptr[lo + 1];
         chunk_start[c] = start;
@@ -467,7 +473,7 @@ __global__ void k_tail_layout(int ntl, const int *lstart, const int *lcnt, int t
             const int G = tail_group(cnt);
             const int sl = (w + G - 1) / G;
             slices[L] = sl;
-            size[L] = sl * sweep_rows_pow2(cnt) * G;
+            size[L] = (sl + kTailBlock - 1) / kTailBlock * kTailBlock * sweep_rows_pow2(cnt) * G;
         }
     }
 }
@@ -482,9 +488,14 @@ __global__ void k_tail_fill(int ntl, const int *lstart, const int *lcnt, int tp0
     for (int L = wave; L < ntl; L += nwaves) {
         const int cnt = lcnt[L], q0 = lstart[L] - tp0;
         const int P = sweep_rows_pow2(cnt), G = tail_group(cnt);
-        const int stride = P * G, total = slices[L] * stride, o = off[L];
+        const int stride = P * G, o = off[L];
+        const int blk = stride * kTailBlock;
+        const int total = (slices[L] + kTailBlock - 1) / kTailBlock * blk;
         for (int idx = lane; idx < total; idx += kWave) {
-            const int sl = idx / stride, t = idx - sl * stride;
+            // idx = block * blk + lane slot * kTailBlock + u: slice block * kTailBlock + u
+            const int b0 = idx / blk, r0 = idx - b0 * blk;
+            const int t = r0 / kTailBlock, u = r0 - t * kTailBlock;
+            const int sl = b0 * kTailBlock + u;
             const int i = t / G, g = t - i * G;
             const int et = sl * G + g;
             int ind = pad_ind;
@@ -512,28 +523,43 @@ __global__ void k_sweep_layout(int nchunks, const int *cnt, const int *width, in
     }
 }
 
+constexpr int kFillWaves = 4;     // waves per k_sweep_fill workgroup (one chunk per wave at a time)
+
 template <typename TS, typename TD>
-__global__ void k_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src,
-                             const int *off, const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val)
+__global__ void __launch_bounds__(kFillWaves * kWave)
+    k_sweep_fill(int nchunks, const int *start, const int *cnt, const int *order, EllSrc src, const int *off,
+                 const int *slices, int pad_ind, TD pad_val, int *ell_ind, TD *ell_val)
 {
+    // source range of every row of the wave's chunk (up to 256 rows)
+    __shared__ int s_rb[kFillWaves][kWaveRows256], s_rl[kFillWaves][kWaveRows256];
     const int lane = lane_id();
+    const int wl = threadIdx.x / kWave;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * blockDim.x / kWave;
     const TS *sval = (const TS *)src.val;
+    int *rbs = s_rb[wl], *rls = s_rl[wl];
     for (int c = wave; c < nchunks; c += nwaves) {
-        const int nc = cnt[c];                  // <= 64 rows (sweep chunks)
+        const int nc = cnt[c];                  // <= 256 rows (sweep chunks)
         const int P = sweep_rows_pow2(nc);
         const int G = sweep_group(nc);
         const int stride = P * G;
         const int total = slices[c] * stride;
         const int o = off[c];
-        // lane i holds row i's source range
-        int rb = 0, rl = 0;
-        if (lane < nc) {
-            const int row = order[start[c] + lane];
-            rb = src.ptr[row];
-            rl = src.ptr[row + 1] - rb;
+        // the previous chunk's reads of the wave's slots are done before they
+        // are overwritten (LDS operations of one wave complete in order; the
+        // fences keep the compiler from moving accesses across)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < nc; i += kWave) {
+            const int row = order[start[c] + i];
+            const int b = src.ptr[row];
+            rbs[i] = b;
+            rls[i] = src.ptr[row + 1] - b;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // four output slots per lane in flight: index, then column and value,
         // then the column's position
         constexpr int U = 4;
@@ -546,9 +572,10 @@ __global__ void k_sweep_fill(int nchunks, const int *start, const int *cnt, cons
                 const int sl = idx / stride, t = idx - sl * stride;
                 const int i = t / G, g = t - i * G;
                 const int et = sl * G + g;
-                const int b = __shfl(rb, i & (kWave - 1), kWave);
-                const int len = __shfl(rl, i & (kWave - 1), kWave);
-                e[u] = (idx < total && i < nc && et < len) ? (src.eidx ? src.eidx[b + et] : b + et) : -1;
+                const bool in = idx < total && i < nc;
+                const int b = in ? rbs[i] : 0;
+                const int len = in ? rls[i] : 0;
+                e[u] = (in && et < len) ? (src.eidx ? src.eidx[b + et] : b + et) : -1;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -686,14 +713,14 @@ void launch_level_width(int ns, const int *order, const int *level, const int *l
 {
     hipLaunchKernelGGL(k_level_width, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, lw);
 }
-void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt, hipStream_t s)
+void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int gmin, int *cnt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_chunk_count, dim3(256), dim3(256), 0, s, nlevels, lvl_ptr, lw, cnt);
+    hipLaunchKernelGGL(k_chunk_count, dim3(256), dim3(256), 0, s, nlevels, lvl_ptr, lw, gmin, cnt);
 }
-void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
+void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw, int gmin,
                        int *chunk_start, int *chunk_cnt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_chunk_fill, dim3(1024), dim3(256), 0, s, nchunks, nlevels, lvl_ptr, chunk_ptr, lw,
+    hipLaunchKernelGGL(k_chunk_fill, dim3(1024), dim3(256), 0, s, nchunks, nlevels, lvl_ptr, chunk_ptr, lw, gmin,
                        chunk_start, chunk_cnt);
 }
 void launch_pack_meta(int nchunks, const int *start, const int *cnt, const int *off, const int *slices, int4 *meta,
@@ -756,8 +783,8 @@ void launch_sweep_fill(int nchunks, const int *start, const int *cnt, const int 
     int g = (nchunks + 3) / 4;
     if (g > 16384) g = 16384;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL((k_sweep_fill<TS, TD>), dim3(g), dim3(256), 0, s, nchunks, start, cnt, order, src, off, slices,
-                       pad_ind, pad_val, ell_ind, ell_val);
+    hipLaunchKernelGGL((k_sweep_fill<TS, TD>), dim3(g), dim3(kFillWaves * kWave), 0, s, nchunks, start, cnt, order,
+                       src, off, slices, pad_ind, pad_val, ell_ind, ell_val);
 }
 
 template void launch_sweep_fill<float, float>(int, const int *, const int *, const int *, EllSrc, const int *,

@@ -148,11 +148,14 @@ void launch_level_bucket_keys(int ns, const int *order, const int *level, const 
                               uint32_t *keys, hipStream_t s);
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s);
 // Rows per sweep chunk of a level whose widest row (of L and of L^T) has
-// `width` entries: G = next power of two >= width / 8 lanes per row (4..64),
-// so a lane holds at most eight entries of a row, and 256 / G rows.
-__host__ __device__ inline int chunk_rows(int width)
+// `width` entries: G = next power of two >= width / 8 lanes per row (gmin..64),
+// so a lane holds at most eight entries of a row, and 256 / G rows (up to 256
+// for gmin = 1: a level of short rows then keeps four times as many rows in
+// flight per workgroup as with G = 4).
+constexpr int kSweepMinGroupDefault = 1;
+__host__ __device__ inline int chunk_rows(int width, int gmin)
 {
-    int g = 4;
+    int g = gmin;
     while (g < 64 && g * 8 < width) g <<= 1;
     return kWaveRows256 / g;
 }
@@ -160,8 +163,8 @@ __host__ __device__ inline int chunk_rows(int width)
 // (lw zeroed by the caller); rows at positions p, level of p = level[order[p]].
 void launch_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw,
                         hipStream_t s);
-void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int *cnt, hipStream_t s);
-void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw,
+void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int gmin, int *cnt, hipStream_t s);
+void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw, int gmin,
                        int *chunk_start, int *chunk_cnt, hipStream_t s);
 // meta[c] = {start[c], cnt[c], off[c], slices[c]}
 void launch_pack_meta(int nchunks, const int *start, const int *cnt, const int *off, const int *slices, int4 *meta,
@@ -192,7 +195,7 @@ void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *o
 // off[c] = 64 * exclusive scan of width (int32); returns via scan kernels.
 void launch_mul64(const int *in, int *out, int n, hipStream_t s);
 
-// Grouped sliced ELL for the triangular sweeps: a chunk of cnt <= 64 rows is
+// Grouped sliced ELL for the triangular sweeps: a chunk of cnt <= 256 rows is
 // processed by one 256-thread workgroup with G = min(256 / P, 64) lanes per
 // row (P = next power of two >= cnt).  Entry t of chunk row i lives in slice
 // s = t / G at off[c] + s * P * G + i * G + (t % G).  slices[c] =
@@ -217,12 +220,17 @@ void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *sli
 // below tp0 ("bulk part": final before the tail starts, folded in by a wide
 // kernel) or at/after tp0 ("internal": read from LDS, column stored as
 // position - tp0).  A tail level of cnt <= kTailMaxRows rows is one group
-// with tail_group(cnt) lanes per row: entry t of level row i lives at
-// off[L] + (t / G) * P * G + i * G + t % G (P = next power of two >= cnt).
-// One extra entry at off[ntl] (index ntail, value 0) is what lanes without a
-// slot load: the workgroup's LDS holds 0 at value slot ntail, so the masked
-// contribution is -0 * 0 and the loaded value needs no arithmetic mask.
+// with G = tail_group(cnt) lanes per row (P = next power of two >= cnt,
+// lane slot i * G + g of the level's P * G) and slices = ceil(width / G)
+// entries per lane slot, stored lane-major in blocks of kTailBlock: entry t
+// of level row i (slice s = t / G, g = t % G) lives at
+//   off[L] + (s / kTailBlock) * P * G * kTailBlock + (i * G + g) * kTailBlock + s % kTailBlock,
+// so a lane loads a block's entries with a few 16-byte loads.  Unused entries
+// are padding (index ntail, value 0); one more block of padding at off[ntl] is
+// what lanes without a slot load.  The workgroup's LDS holds 0 at value slot
+// ntail, so a padded contribution is -0 * 0 and needs no arithmetic mask.
 constexpr int kTailMaxRows = 1024;
+constexpr int kTailBlock = 8;
 __host__ __device__ inline int tail_group(int cnt)
 {
     int g = kTailThreads / sweep_rows_pow2(cnt);
@@ -239,7 +247,8 @@ void launch_tail_count(int ntail, int tp0, const int *order, EllSrc src, int *cn
 template <typename TD>
 void launch_tail_split(int ntail, int tp0, const int *order, EllSrc src, const int *ptr_b, const int *ptr_i,
                        int *ind_b, TD *val_b, int *ind_i, TD *val_i, hipStream_t s);
-// slices[L] = ceil(max internal row length of level L / G), size[L] = slices * P * G
+// slices[L] = ceil(max internal row length of level L / G),
+// size[L] = ceil(slices / kTailBlock) * kTailBlock * P * G
 void launch_tail_layout(int ntl, const int *lstart, const int *lcnt, int tp0, const int *ptr_i, int *slices,
                         int *size, hipStream_t s);
 template <typename TD>

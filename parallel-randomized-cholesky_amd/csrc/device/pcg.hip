@@ -535,7 +535,33 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-constexpr int kTailPref = 8;    // slices per lane loaded before the LDS reads
+constexpr int kTailPref = kTailBlock;    // slices per lane loaded before the LDS reads
+
+// The kTailBlock consecutive entries of a lane slot (16-byte aligned: every
+// level's block size is a multiple of kTailBlock entries).
+template <typename T>
+__device__ __forceinline__ void tail_block_load(const int *ind, const T *val, int base, int *j, T *l)
+{
+    static_assert(kTailBlock == 8, "two 16-byte index loads per block");
+    const int4 *pi = reinterpret_cast<const int4 *>(ind + base);
+    const int4 i0 = pi[0], i1 = pi[1];
+    j[0] = i0.x; j[1] = i0.y; j[2] = i0.z; j[3] = i0.w;
+    j[4] = i1.x; j[5] = i1.y; j[6] = i1.z; j[7] = i1.w;
+    if constexpr (sizeof(T) == 4) {
+        const float4 *pv = reinterpret_cast<const float4 *>(val + base);
+        const float4 v0 = pv[0], v1 = pv[1];
+        l[0] = v0.x; l[1] = v0.y; l[2] = v0.z; l[3] = v0.w;
+        l[4] = v1.x; l[5] = v1.y; l[6] = v1.z; l[7] = v1.w;
+    } else {
+        const double2 *pv = reinterpret_cast<const double2 *>(val + base);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const double2 v = pv[h];
+            l[2 * h] = v.x;
+            l[2 * h + 1] = v.y;
+        }
+    }
+}
 
 template <typename T, bool kFwd>
 __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
@@ -571,7 +597,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     if (t == 0) v[nt] = (T)0;        // slot of the padding entries
     // level descriptors {start, cnt, ELL offset, slices} after the values
     const int ntl = a.ntail_levels;
-    const int ze = off[ntl];          // the zero entry after the last level
+    const int ze = off[ntl];          // the padding block after the last level
     int4 *tm = reinterpret_cast<int4 *>(reinterpret_cast<char *>(tail_lds_d) + tail_value_bytes<T>(nt));
     for (int L = t; L < ntl; L += kTailThreads) tm[L] = make_int4(a.tl_start[L], a.tl_cnt[L], off[L], slices[L]);
     // an empty level (no rows, no slices) after the last one: the loop below
@@ -586,17 +612,14 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     auto fetch = [&](int s, int4 &d, int *j, T *l) {
         d = tm[s < ntl ? (kFwd ? s : ntl - 1 - s) : ntl];
         const int stride = sweep_rows_pow2(d.y) * tail_group(d.y);
-#pragma unroll
-        for (int u = 0; u < kTailPref; ++u) {
-            // every lane loads (the zero entry when it has no slice u), so the
-            // loads are never predicated away, and nothing is computed from
-            // the loaded values before their level is solved (a mask applied
-            // here would make the compiler wait for these loads at once)
-            const bool has = t < stride && u < d.w;
-            const int e = has ? d.z + t + u * stride : ze;
-            j[u] = ind[e];
-            l[u] = val[e];
-        }
+        // every lane loads (the padding block when it has no slot), so the
+        // loads are never predicated away, and nothing is computed from the
+        // loaded values before their level is solved (a mask applied here
+        // would make the compiler wait for these loads at once); slices past
+        // the level's own are padding in the layout.  A level without
+        // internal entries (d.w = 0, also the empty one past the end) has no
+        // block of its own.
+        tail_block_load<T>(ind, val, (t < stride && d.w > 0) ? d.z + t * kTailBlock : ze, j, l);
     };
     auto solve = [&](const int4 &d, const int *j, const T *l) {
         const int G = tail_group(d.y);
@@ -608,21 +631,14 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
 #pragma unroll
         for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
         if (t < stride) {
-            // slices past the prefetched ones (long rows), eight loads in
-            // flight at a time; a masked slot (the zero entry) adds
+            // blocks past the prefetched one (long rows); a padded entry adds
             // -0 * 0 = -0, which leaves acc unchanged
-            for (int sl = kTailPref; sl < d.w; sl += 8) {
-                int jj[8];
-                T ll[8];
+            for (int sl = kTailPref; sl < d.w; sl += kTailBlock) {
+                int jj[kTailBlock];
+                T ll[kTailBlock];
+                tail_block_load<T>(ind, val, d.z + sl * stride + t * kTailBlock, jj, ll);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const bool has = sl + u < d.w;
-                    const int e = has ? d.z + t + (sl + u) * stride : ze;
-                    jj[u] = ind[e];
-                    ll[u] = val[e];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
+                for (int u = 0; u < kTailBlock; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
             }
         }
         if (lead) acc += v[q];

@@ -116,6 +116,11 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (sc) h->seg_cap_limit = std::max(1, std::atoi(sc));
         const char *at = std::getenv("RCHOL_ASYNC_TAIL");       // 0: narrow rounds stay round kernels
         if (at) h->async_tail = std::atoi(at) != 0;
+        const char *mg = std::getenv("RCHOL_SWEEP_MIN_GROUP");  // lanes per row of the widest sweep chunks (1, 2, 4)
+        if (mg) {
+            const int v = std::atoi(mg);
+            h->sweep_gmin = v <= 1 ? 1 : (v <= 2 ? 2 : 4);
+        }
     }
     *out = h;
     return RC_OK;
@@ -888,7 +893,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(hipMemsetAsync(h->lvl_w.p, 0, nlsz * sizeof(int), s));
     launch_level_width(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(), h->lt_ptr.as<int>(),
                        h->lvl_w.as<int>(), s);
-    launch_chunk_count(nl, lptr, h->lvl_w.as<int>(), lcnt, s);
+    launch_chunk_count(nl, lptr, h->lvl_w.as<int>(), h->sweep_gmin, lcnt, s);
     scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
     int nchunks = 0;
     RC_HIP(hipMemcpyAsync(&nchunks, cptr + nl, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -896,7 +901,8 @@ static int assemble(rc_handle_t h, double *ms_out)
     h->nchunks = nchunks;
     RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
-    launch_chunk_fill(nchunks, nl, lptr, cptr, h->lvl_w.as<int>(), h->chunk_start.as<int>(), h->chunk_cnt.as<int>(), s);
+    launch_chunk_fill(nchunks, nl, lptr, cptr, h->lvl_w.as<int>(), h->sweep_gmin, h->chunk_start.as<int>(),
+                      h->chunk_cnt.as<int>(), s);
     // level split: the trailing run of levels with fewer than tail_max_rows
     // rows, as long as their rows fit the tail workgroup's LDS, is the tail
     std::vector<int> hl(nlsz), hc(nlsz);
@@ -1038,15 +1044,20 @@ static int assemble(rc_handle_t h, double *ms_out)
             int total = 0;
             RC_HIP(hipMemcpyAsync(&total, off.as<int>() + ntl, sizeof(int), hipMemcpyDeviceToHost, s));
             RC_HIP(hipStreamSynchronize(s));
-            // one more entry after the levels: the zero slot masked lanes load
-            RC_HIP(ind.ensure(((size_t)total + 1) * sizeof(int)));
-            RC_HIP(val.ensure(((size_t)total + 1) * sizeof(T)));
+            // one more block after the levels: the padding lanes without a slot load
+            RC_HIP(ind.ensure(((size_t)total + kTailBlock) * sizeof(int)));
+            RC_HIP(val.ensure(((size_t)total + kTailBlock) * sizeof(T)));
             launch_tail_fill<T>(ntl, h->tl_start.as<int>(), h->tl_cnt.as<int>(), tp0, h->ti_ptr.as<int>(),
                                 h->ti_ind.as<int>(), h->ti_val.as<T>(), off.as<int>(), slices.as<int>(), nt,
                                 ind.as<int>(), val.as<T>(), s);
-            const T zero_val = (T)0;
-            RC_HIP(hipMemcpyAsync(ind.as<int>() + total, &nt, sizeof(int), hipMemcpyHostToDevice, s));
-            RC_HIP(hipMemcpyAsync(val.as<T>() + total, &zero_val, sizeof(T), hipMemcpyHostToDevice, s));
+            int pad_ind[kTailBlock];
+            T pad_val[kTailBlock];
+            for (int u = 0; u < kTailBlock; ++u) {
+                pad_ind[u] = nt;
+                pad_val[u] = (T)0;
+            }
+            RC_HIP(hipMemcpyAsync(ind.as<int>() + total, pad_ind, sizeof(pad_ind), hipMemcpyHostToDevice, s));
+            RC_HIP(hipMemcpyAsync(val.as<T>() + total, pad_val, sizeof(pad_val), hipMemcpyHostToDevice, s));
             RC_HIP(hipStreamSynchronize(s));
             return RC_OK;
         };

@@ -249,6 +249,32 @@ def test_level_orders_agree_with_oracle(rc, orc, wl, order, prec, monkeypatch):
     assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("gmin", ["1", "2", "4"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_sweep_chunk_sizes_agree_with_oracle(rc, orc, wl, gmin, prec, monkeypatch):
+    """Sweep chunks of up to 256 rows (one lane per row of a level whose rows
+    have at most eight entries), 128 rows, or 64 rows (the narrowest groups):
+    the same answer as the oracle.  A 2D grid has short rows in its wide
+    levels, so the 256- and 128-row chunks occur."""
+    say(f"[gpu] sweep chunk min group {gmin} prec={prec}")
+    monkeypatch.setenv("RCHOL_SWEEP_MIN_GROUP", gmin)
+    monkeypatch.setenv("RCHOL_TAIL_ROWS", "64")
+    n, r, c, v = wl.grid_laplacian((120, 110), "uniform_0.5_2", seed=4)
+    s = rc.Solver(device=0, seed=3, precision=prec)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    s.factor()
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    fac, sys_, perm, _ = orc.prepare_system(n, r, c, v, order="nnz-sort", order_seed=1)
+    f = orc.factor(fac, seed=3)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    tol = 1e-7 if prec == 64 else 1e-4
+    assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
+    s.close()
+
+
 def test_autotune_keeps_the_solution(rc, orc, wl):
     say("[gpu] autotune")
     s, data, kind = _solver(rc, wl, "grid3d_20_log", 32)
