@@ -212,6 +212,21 @@ __global__ void k_level_bucket_keys(int ns, const int *order, const int *level, 
     }
 }
 
+// keys of rows order[q]: level << 2 | sweep-group class, class c = the number
+// of doublings from one lane per row that a row of width w needs to hold at
+// most eight entries per lane (w <= 8: 0, <= 16: 1, <= 32: 2, else 3)
+__global__ void k_level_group_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                                   uint32_t *keys)
+{
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ns; q += gridDim.x * blockDim.x) {
+        const int row = order[q];
+        const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
+        const int w = a > b ? a : b;
+        const int cls = w <= 8 ? 0 : (w <= 16 ? 1 : (w <= 32 ? 2 : 3));
+        keys[q] = ((uint32_t)level[row] << 2) | (uint32_t)cls;
+    }
+}
+
 __global__ void k_inverse_perm(int n, const int *order, int *pos_of)
 {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) pos_of[order[p]] = p;
@@ -311,7 +326,8 @@ __global__ void k_ell_fill(int nslices, const int *start, const int *cnt, const 
 // changes (a wide level's words would otherwise take one returning atomic per
 // 64 rows, all on one address).
 constexpr int kLevelSpan = 1024;
-__global__ void k_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw)
+__global__ void k_level_width(int ns, const int *order, const int *level, const uint32_t *seg_keys, int seg_shift,
+                              const int *l_ptr, const int *lt_ptr, int *lw)
 {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int nwaves = gridDim.x * blockDim.x / kWave;
@@ -323,7 +339,7 @@ __global__ void k_level_width(int ns, const int *order, const int *level, const 
             int L = -1, w = 0;
             if (p < ns) {
                 const int row = order[p];
-                L = level[row];
+                L = seg_keys ? (int)(seg_keys[p] >> seg_shift) : level[row];
                 const int a = l_ptr[row + 1] - l_ptr[row], b = lt_ptr[row + 1] - lt_ptr[row];
                 w = a > b ? a : b;
             }
@@ -671,6 +687,11 @@ void launch_level_bucket_keys(int ns, const int *order, const int *level, const 
 {
     hipLaunchKernelGGL(k_level_bucket_keys, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, keys);
 }
+void launch_level_group_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                             uint32_t *keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_level_group_keys, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, keys);
+}
 void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s)
 {
     hipLaunchKernelGGL(k_inverse_perm, dim3(2048), dim3(256), 0, s, n, order, pos_of);
@@ -708,10 +729,11 @@ void launch_ell_fill(int nslices, const int *start, const int *cnt, const int *o
                        width, pad_ind, pad_val, ell_ind, ell_val);
 }
 
-void launch_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw,
-                        hipStream_t s)
+void launch_level_width(int ns, const int *order, const int *level, const uint32_t *seg_keys, int seg_shift,
+                        const int *l_ptr, const int *lt_ptr, int *lw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_level_width, dim3(2048), dim3(256), 0, s, ns, order, level, l_ptr, lt_ptr, lw);
+    hipLaunchKernelGGL(k_level_width, dim3(2048), dim3(256), 0, s, ns, order, level, seg_keys, seg_shift, l_ptr, lt_ptr,
+                       lw);
 }
 void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int gmin, int *cnt, hipStream_t s)
 {

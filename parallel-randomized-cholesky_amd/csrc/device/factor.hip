@@ -38,6 +38,7 @@ namespace rc {
 namespace {
 
 constexpr int kLdsCap = 128;                                   // entries per wave in LDS (round kernel)
+static_assert(kLdsCap <= 2 * 64, "the round kernel sorts its columns in registers (two slots per lane)");
 constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc jt
 constexpr long long kFillChunk = 2048;
 constexpr long long kOutChunk = 2048;
@@ -73,31 +74,61 @@ __device__ __forceinline__ int next_pow2(int v)
 // every merge stage); every comparator puts the minimum at the lower index, so
 // positions >= len behave as +infinity and need no padding.
 // kByRow: key (row, w); otherwise key (w, row).  pay is carried if non-null.
-template <bool kByRow>
-__device__ __forceinline__ void wave_sort(int *row, double *w, int *pay, int len)
+// The comparators of a stage touch disjoint pairs, so each lane reads all of
+// its pairs of the stage (kNB = P / 128 of them, at most 8 at a time) before
+// it compares and writes any: one LDS round trip per stage instead of one per
+// 64 comparators.  kNB is a template argument so no slot is computed idly.
+template <bool kByRow, int kNB>
+__device__ __forceinline__ void wave_sort_nb(int *row, double *w, int *pay, int len, int P)
 {
     const int lane = lane_id();
-    const int P = next_pow2(len);
+    const int half = P >> 1;
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < (P >> 1); t += kWave) {
-                int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                int l = (j == (k >> 1)) ? (i ^ (k - 1)) : (i + j);
-                if (l < len) {
-                    int ri = row[i], rl = row[l];
-                    double wi = w[i], wl = w[l];
-                    bool gt = kByRow ? (ri > rl || (ri == rl && wi > wl))
-                                     : (wi > wl || (wi == wl && ri > rl));
-                    if (gt) {
-                        row[i] = rl; row[l] = ri;
-                        w[i] = wl; w[l] = wi;
-                        if (pay) { int mi = pay[i]; pay[i] = pay[l]; pay[l] = mi; }
+            for (int t0 = 0; t0 < half; t0 += kNB * kWave) {
+                int ia[kNB], la[kNB], ri[kNB], rl[kNB], pi[kNB], pl[kNB];
+                double wi[kNB], wl[kNB];
+                bool act[kNB];
+#pragma unroll
+                for (int b = 0; b < kNB; ++b) {
+                    const int t = t0 + b * kWave + lane;
+                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    const int l = (j == (k >> 1)) ? (i ^ (k - 1)) : (i + j);
+                    act[b] = l < len;                      // t < half always; l > i, so i < len as well
+                    ia[b] = i;
+                    la[b] = l;
+                    // read unconditionally (i, l < P <= the buffer's capacity),
+                    // so no branch sits between the reads of the slots
+                    ri[b] = row[i];
+                    rl[b] = row[l];
+                    wi[b] = w[i];
+                    wl[b] = w[l];
+                    pi[b] = pay ? pay[i] : 0;
+                    pl[b] = pay ? pay[l] : 0;
+                }
+#pragma unroll
+                for (int b = 0; b < kNB; ++b) {
+                    const bool gt = kByRow ? (ri[b] > rl[b] || (ri[b] == rl[b] && wi[b] > wl[b]))
+                                           : (wi[b] > wl[b] || (wi[b] == wl[b] && ri[b] > rl[b]));
+                    if (act[b] && gt) {
+                        row[ia[b]] = rl[b]; row[la[b]] = ri[b];
+                        w[ia[b]] = wl[b]; w[la[b]] = wi[b];
+                        if (pay) { pay[ia[b]] = pl[b]; pay[la[b]] = pi[b]; }
                     }
                 }
             }
             __syncthreads();
         }
     }
+}
+// Networks of P >= 256 positions (shorter columns use the register sorts).
+template <bool kByRow>
+__device__ __forceinline__ void wave_sort(int *row, double *w, int *pay, int len)
+{
+    const int P = next_pow2(len);
+    if (P <= 256) wave_sort_nb<kByRow, 2>(row, w, pay, len, 256);
+    else if (P <= 512) wave_sort_nb<kByRow, 4>(row, w, pay, len, 512);
+    else wave_sort_nb<kByRow, 8>(row, w, pay, len, P);
 }
 
 // The same ordering for len <= 64 R entries held in registers, entry i in
@@ -174,9 +205,17 @@ __device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len
 // kMaxR bounds the register slots (and so the kernel's register budget).  Both
 // kernels use 2: above 128 entries the LDS network measured faster than
 // 4 or 8 slots per lane.
-template <bool kByRow, int kMaxR>
+// kShort: the caller guarantees len <= kMaxR * 64 (the round kernel defers
+// longer columns), so the LDS network is not compiled in (its registers would
+// lower the round kernel's occupancy).
+template <bool kByRow, int kMaxR, bool kShort = false>
 __device__ __forceinline__ void column_sort(int *row, double *w, int *pay, int len)
 {
+    if constexpr (kShort) {
+        if (len <= kWave) lane_sort<kByRow, 1>(row, w, pay, len);
+        else lane_sort<kByRow, kMaxR>(row, w, pay, len);
+        return;
+    }
     if (len <= kWave) lane_sort<kByRow, 1>(row, w, pay, len);
     else if (kMaxR >= 2 && len <= 2 * kWave) lane_sort<kByRow, (kMaxR >= 2 ? 2 : 1)>(row, w, pay, len);
     else if (kMaxR >= 4 && len <= 4 * kWave) lane_sort<kByRow, (kMaxR >= 4 ? 4 : 1)>(row, w, pay, len);
@@ -305,14 +344,14 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
 // Steps 2-5 of a column whose m >= 1 entries are gathered in b.  Inlined at
 // every call so that b's accesses compile to LDS or global instructions (a
 // generic pointer would make each LDS access wait for all memory traffic).
-template <int kMaxR, bool kAsync>
+template <int kMaxR, bool kAsync, bool kShort = false>
 __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int round, const ColBuf &b, int m,
                                                Cursor &fcur, Cursor &ocur, int *next, unsigned int *next_set)
 {
     const int lane = lane_id();
     const int last = a.n - 1;
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
-    column_sort<true, kMaxR>(b.srow, b.sw, b.smult, m);
+    column_sort<true, kMaxR, kShort>(b.srow, b.sw, b.smult, m);
     int d = 0;
     for (int base_i = 0; base_i < m; base_i += kWave) {
         int i = base_i + lane;
@@ -351,7 +390,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
         b.smult[t] = t;
     }
     __syncthreads();
-    column_sort<false, kMaxR>(b.srow, b.sw, b.smult, d);
+    column_sort<false, kMaxR, kShort>(b.srow, b.sw, b.smult, d);
     // canonical blocked scan (shared with the oracle): per 64-entry block the
     // doubling scan x_i += x_{i-off}, off = 1..32, then the previous block's
     // last sum is added
@@ -408,96 +447,111 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
     }
     __syncthreads();   // minc and jt complete
 
-    if constexpr (kAsync) {
-        // ---- 5 (queue-driven tail). A neighbour may be taken by another wave
-        //      as soon as its dependency count reaches zero, so every fill push,
-        //      factor entry and level of this column is made visible (release
-        //      fence) before any dependency update; ready neighbours go to the
-        //      queue after an acquire-release fence (the other columns' pushes
-        //      into them are then visible to whoever takes them).
-        const int lk = round + 1;
-        for (int base_i = 0; base_i < d; base_i += kWave) {
-            const int e = base_i + lane;
+    // ---- 5. per block of 64 entries: the neighbour's net dependency update
+    //         (:653-687), the fill edge (max, min, w_i (S - C_i) / S) pushed
+    //         onto column min's lists, and the factor entry -w/S in row order.
+    //         Up to kB blocks are handled together: their returning atomics
+    //         (list-head exchanges, dependency adds) are all in flight at once,
+    //         so a long column pays one memory round trip per kB blocks, not
+    //         one per block.  The queue-driven tail makes every push, entry and
+    //         raised level visible (release fence) before any dependency add,
+    //         since a neighbour may be taken by another wave as soon as its
+    //         count reaches zero.
+    constexpr int kB = kShort ? 2 : 8;
+    for (int sb = 0; sb < d; sb += kB * kWave) {
+        // the returning atomics of the batch: list-head exchanges and (rounds
+        // only: nothing orders them there but the kernel boundary) the
+        // dependency adds, all in flight together
+        int old[kB], r[kB], nd[kB];
+        bool upd[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int e = sb + u * kWave + lane;
+            old[u] = -1;
+            r[u] = -1;
+            upd[u] = false;
+            nd[u] = 1;
+            if (e < nfill_out) {
+                const int j = b.jt[e];
+                const int ri = b.srow[e], rj = b.srow[j];
+                const int cc = ri > rj ? rj : ri;
+                const int rec = (int)(fbase + e);
+                old[u] = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
+            }
+            if (!kAsync && e < d) {
+                r[u] = b.mrow[e];
+                const int delta = b.minc[e] - b.mmult[e];
+                upd[u] = r[u] != last && delta != 0;
+                if (upd[u]) nd[u] = atomicAdd(&a.dep[r[u]], delta) + delta;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int e = sb + u * kWave + lane;
             if (e < d) {
-                const int r = b.mrow[e];
-                a.out_row[obase + e] = r;
+                const int rr = b.mrow[e];
+                a.out_row[obase + e] = rr;
                 a.out_val[obase + e] = -(b.mw[e] / S);
-                if (r != last) atomicMax(&a.level[r], lk);
+                if (kAsync && rr != last) atomicMax(&a.level[rr], round + 1);
             }
             if (e < nfill_out) {
                 const int j = b.jt[e];
                 const int ri = b.srow[e], rj = b.srow[j];
-                const int rr = ri > rj ? ri : rj;
-                const int cc = ri > rj ? rj : ri;
-                const double rest = S - b.C[e];
-                const double fw = (b.sw[e] * rest) / S;
-                const int rec = (int)(fbase + e);
                 FillRec fr;
-                fr.row = rr;
-                fr.w = fw;
-                fr.next = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
-                a.pool[rec] = fr;
+                fr.row = ri > rj ? ri : rj;
+                fr.next = old[u];
+                fr.w = (b.sw[e] * (S - b.C[e])) / S;
+                a.pool[(int)(fbase + e)] = fr;
             }
         }
+        if constexpr (!kAsync) {
+#pragma unroll
+            for (int u = 0; u < kB; ++u) list_append(a, next_set, next, upd[u] && nd[u] == 0, r[u]);
+        }
+    }
+    if constexpr (kAsync) {
         fence_release();
-        for (int base_i = 0; base_i < d; base_i += kWave) {
-            const int e = base_i + lane;
-            bool ready = false;
-            int r = -1;
-            if (e < d) {
-                r = b.mrow[e];
-                const int delta = b.minc[e] - b.mmult[e];
-                if (r != last && delta != 0) ready = atomicAdd(&a.dep[r], delta) + delta == 0;
+        for (int sb = 0; sb < d; sb += kB * kWave) {
+            int r[kB], nd[kB];
+            bool upd[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int e = sb + u * kWave + lane;
+                r[u] = -1;
+                upd[u] = false;
+                nd[u] = 1;
+                if (e < d) {
+                    r[u] = b.mrow[e];
+                    const int delta = b.minc[e] - b.mmult[e];
+                    upd[u] = r[u] != last && delta != 0;
+                    if (upd[u]) nd[u] = atomicAdd(&a.dep[r[u]], delta) + delta;
+                }
             }
             fence_acq_rel();
-            const uint64_t mask = ballot(ready);
-            if (mask) {
+            // the ready neighbours of these blocks, one queue atomic for all
+            unsigned int cnt = 0;
+            uint64_t mk[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                mk[u] = ballot(upd[u] && nd[u] == 0);
+                cnt += (unsigned int)__popcll(mk[u]);
+            }
+            if (cnt) {
                 unsigned int qb = 0;
-                if (lane == 0) qb = atomicAdd(a.aq_ctr + kAqTail, (unsigned int)__popcll(mask));
+                if (lane == 0) qb = atomicAdd(a.aq_ctr + kAqTail, cnt);
                 qb = bcast(qb, 0);
-                // every remaining column is queued once: a slot past the
-                // queue's end means inconsistent dependency counts
-                const unsigned int slot = qb + (unsigned int)lanes_below(mask);
-                if (ready && slot < a.aq_total) st_relaxed(&next[slot], r);
-                if (ready && slot >= a.aq_total) raise_status(a.ctr, kStatusInconsistent);
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    // every remaining column is queued once: a slot past the
+                    // queue's end means inconsistent dependency counts
+                    const bool ready = (mk[u] >> lane) & 1ull;
+                    const unsigned int slot = qb + (unsigned int)lanes_below(mk[u]);
+                    if (ready && slot < a.aq_total) st_relaxed(&next[slot], r[u]);
+                    if (ready && slot >= a.aq_total) raise_status(a.ctr, kStatusInconsistent);
+                    qb += (unsigned int)__popcll(mk[u]);
+                }
             }
         }
-        __syncthreads();
-        return true;
-    }
-
-    // ---- 5. per 64 entries: the neighbour's net dependency update (:653-687),
-    //         the fill edge (max, min, w_i (S - C_i) / S) pushed onto column
-    //         min's lists, and the factor entry -w/S in row order; the
-    //         dependency add and the list push are in flight together
-    for (int base_i = 0; base_i < d; base_i += kWave) {
-        const int e = base_i + lane;
-        bool upd = false;
-        int r = -1, delta = 0, old_dep = 0;
-        if (e < d) {
-            r = b.mrow[e];
-            delta = b.minc[e] - b.mmult[e];
-            upd = r != last && delta != 0;
-            if (upd) old_dep = atomicAdd(&a.dep[r], delta);
-            a.out_row[obase + e] = r;
-            a.out_val[obase + e] = -(b.mw[e] / S);
-        }
-        if (e < nfill_out) {
-            const int j = b.jt[e];
-            const int ri = b.srow[e], rj = b.srow[j];
-            const int rr = ri > rj ? ri : rj;
-            const int cc = ri > rj ? rj : ri;
-            const double rest = S - b.C[e];
-            const double fw = (b.sw[e] * rest) / S;
-            const int rec = (int)(fbase + e);
-            const int old = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
-            FillRec fr;
-            fr.row = rr;
-            fr.next = old;
-            fr.w = fw;
-            a.pool[rec] = fr;
-        }
-        list_append(a, next_set, next, upd && old_dep + delta == 0, r);
     }
     __syncthreads();
     return true;
@@ -555,7 +609,8 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
             return process_column<kMaxR, kAsync>(a, k, round, g, m, fcur, ocur, next, next_set);
         }
     }
-    return process_column<kMaxR, kAsync>(a, k, round, b, m, fcur, ocur, next, next_set);
+    // the round kernel (kDefer) reaches here only with P <= lds_cap = kLdsCap entries
+    return process_column<kMaxR, kAsync, kDefer>(a, k, round, b, m, fcur, ocur, next, next_set);
 }
 
 }  // namespace

@@ -146,6 +146,9 @@ void launch_inverse_perm(int n, const int *order, int *pos_of, hipStream_t s);
 // keys of rows order[q]: level << 3 | width class (see assemble.hip)
 void launch_level_bucket_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
                               uint32_t *keys, hipStream_t s);
+// keys of rows order[q]: level << 2 | sweep-group class (see assemble.hip)
+void launch_level_group_keys(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr,
+                             uint32_t *keys, hipStream_t s);
 void launch_level_max(int ns, const int *level, unsigned long long *out, hipStream_t s);
 // Rows per sweep chunk of a level whose widest row (of L and of L^T) has
 // `width` entries: G = next power of two >= width / 8 lanes per row (gmin..64),
@@ -159,10 +162,11 @@ __host__ __device__ inline int chunk_rows(int width, int gmin)
     while (g < 64 && g * 8 < width) g <<= 1;
     return kWaveRows256 / g;
 }
-// lw[L] = max over the rows of level L of max(len L row, len L^T row)
-// (lw zeroed by the caller); rows at positions p, level of p = level[order[p]].
-void launch_level_width(int ns, const int *order, const int *level, const int *l_ptr, const int *lt_ptr, int *lw,
-                        hipStream_t s);
+// lw[L] = max over the rows of segment L of max(len L row, len L^T row)
+// (lw zeroed by the caller); rows at positions p, segment of p =
+// seg_keys[p] >> seg_shift (sorted keys) or, without keys, level[order[p]].
+void launch_level_width(int ns, const int *order, const int *level, const uint32_t *seg_keys, int seg_shift,
+                        const int *l_ptr, const int *lt_ptr, int *lw, hipStream_t s);
 void launch_chunk_count(int nlevels, const int *lvl_ptr, const int *lw, int gmin, int *cnt, hipStream_t s);
 void launch_chunk_fill(int nchunks, int nlevels, const int *lvl_ptr, const int *chunk_ptr, const int *lw, int gmin,
                        int *chunk_start, int *chunk_cnt, hipStream_t s);

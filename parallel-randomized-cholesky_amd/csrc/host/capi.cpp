@@ -108,6 +108,7 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (lo && std::string(lo) == "length") h->level_order = 0;
         if (lo && std::string(lo) == "spatial") h->level_order = 1;
         if (lo && std::string(lo) == "bucket") h->level_order = 2;
+        if (lo && std::string(lo) == "groups") h->level_order = 3;
         const char *nm = std::getenv("RCHOL_NAP_MAX");          // polling back-off cap (1..64)
         if (nm) h->nap_max = std::max(1, std::min(64, std::atoi(nm)));
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
@@ -855,6 +856,16 @@ static int assemble(rc_handle_t h, double *ms_out)
         launch_iota(h->order.as<int>(), ns, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + sys_bits + len_bits,
                          h->tmp_b.as<int>(), s);
+    } else if (h->level_order == 3 && lvl_bits + 2 <= 32) {
+        // rows of a level in four classes by the lanes per row their width
+        // needs, each class in the loaded (user) order: short rows get their
+        // own (wider) chunks instead of the chunk size of the level's widest
+        // row, and neighbouring rows of a chunk stay neighbours in the input
+        launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        launch_level_group_keys(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(), h->lt_ptr.as<int>(),
+                                k0, s);
+        radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + 2, h->tmp_b.as<int>(), s);
+        len_bits = 2;
     } else if (h->level_order == 2 && lvl_bits + 3 <= 32) {
         // rows of a level in classes of similar width (log2 of the wider of
         // the L and L^T rows, widest first), each class in the loaded (user)
@@ -884,32 +895,47 @@ static int assemble(rc_handle_t h, double *ms_out)
     // forward layouts read it directly instead of mapping per entry
     launch_row_expand(ns, h->lt_ptr.as<int>(), h->pos_of.as<int>(), h->rowk.as<int>(), s);
     const size_t nlsz = (size_t)nl + 1;
-    int *lptr = h->tmp_a.as<int>();
-    int *lcnt = lptr + nlsz;
-    int *cptr = lcnt + nlsz;
-    launch_segment_bounds(ns, k0, len_bits + sys_bits, nl, lptr, s);
-    // chunk size per level from its widest row (at most 8 entries per lane)
-    RC_HIP(h->lvl_w.ensure(nlsz * sizeof(int)));
-    RC_HIP(hipMemsetAsync(h->lvl_w.p, 0, nlsz * sizeof(int), s));
-    launch_level_width(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(), h->lt_ptr.as<int>(),
-                       h->lvl_w.as<int>(), s);
-    launch_chunk_count(nl, lptr, h->lvl_w.as<int>(), h->sweep_gmin, lcnt, s);
-    scan_exclusive(lcnt, cptr, nl, h->tmp_scan.as<int>(), s);
+    // chunk segments: the levels, or, in the width-class order, every
+    // (level, class) run of a level (8 per level), so each class gets the
+    // chunk size of its own widest row rather than of the level's
+    const bool classes = sys_bits == 0 && ((h->level_order == 2 && len_bits == 3) || (h->level_order == 3 && len_bits == 2));
+    const int per_level = classes ? 1 << len_bits : 1;
+    const int nseg = nl * per_level;
+    const int seg_shift = classes ? 0 : len_bits + sys_bits;
+    const size_t nssz = (size_t)nseg + 1;
+    RC_HIP(h->seg_work.ensure(3 * nssz * sizeof(int)));
+    int *lptr = h->seg_work.as<int>();
+    int *lcnt = lptr + nssz;
+    int *cptr = lcnt + nssz;
+    launch_segment_bounds(ns, k0, seg_shift, nseg, lptr, s);
+    // chunk size per segment from its widest row (at most 8 entries per lane)
+    RC_HIP(h->lvl_w.ensure(nssz * sizeof(int)));
+    RC_HIP(hipMemsetAsync(h->lvl_w.p, 0, nssz * sizeof(int), s));
+    launch_level_width(ns, h->order.as<int>(), h->level.as<int>(), classes ? k0 : nullptr, seg_shift,
+                       h->l_ptr.as<int>(), h->lt_ptr.as<int>(), h->lvl_w.as<int>(), s);
+    RC_HIP(h->tmp_scan.ensure(scan_tmp_elems(nseg + 2) * sizeof(int) + 64));
+    launch_chunk_count(nseg, lptr, h->lvl_w.as<int>(), h->sweep_gmin, lcnt, s);
+    scan_exclusive(lcnt, cptr, nseg, h->tmp_scan.as<int>(), s);
     int nchunks = 0;
-    RC_HIP(hipMemcpyAsync(&nchunks, cptr + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+    RC_HIP(hipMemcpyAsync(&nchunks, cptr + nseg, sizeof(int), hipMemcpyDeviceToHost, s));
     RC_HIP(hipStreamSynchronize(s));
     h->nchunks = nchunks;
     RC_HIP(h->chunk_start.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
     RC_HIP(h->chunk_cnt.ensure((size_t)std::max(nchunks, 1) * sizeof(int)));
-    launch_chunk_fill(nchunks, nl, lptr, cptr, h->lvl_w.as<int>(), h->sweep_gmin, h->chunk_start.as<int>(),
+    launch_chunk_fill(nchunks, nseg, lptr, cptr, h->lvl_w.as<int>(), h->sweep_gmin, h->chunk_start.as<int>(),
                       h->chunk_cnt.as<int>(), s);
     // level split: the trailing run of levels with fewer than tail_max_rows
     // rows, as long as their rows fit the tail workgroup's LDS, is the tail
     std::vector<int> hl(nlsz), hc(nlsz);
     {
-        RC_HIP(hipMemcpyAsync(hl.data(), lptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
-        RC_HIP(hipMemcpyAsync(hc.data(), cptr, nlsz * sizeof(int), hipMemcpyDeviceToHost, s));
+        std::vector<int> hs(nssz), hcs(nssz);
+        RC_HIP(hipMemcpyAsync(hs.data(), lptr, nssz * sizeof(int), hipMemcpyDeviceToHost, s));
+        RC_HIP(hipMemcpyAsync(hcs.data(), cptr, nssz * sizeof(int), hipMemcpyDeviceToHost, s));
         RC_HIP(hipStreamSynchronize(s));
+        for (size_t L = 0; L < nlsz; ++L) {       // level L starts at its first segment
+            hl[L] = hs[L * (size_t)per_level];
+            hc[L] = hcs[L * (size_t)per_level];
+        }
         if (h->tail_lds_cap < 0) h->tail_lds_cap = tail_lds_limit(h->opt.device);
         int tl = nl;
         int64_t rows = 0;

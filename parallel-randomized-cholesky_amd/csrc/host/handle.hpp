@@ -109,6 +109,7 @@ struct rc_handle_s {
     DevVec lt_ptr, lt_ind, lt_val, diagT;
     // solve layout: level-order positions, sliced ELL of L, L^T and A
     DevVec order, pos_of, idx_user, chunk_start, chunk_cnt, diagP, lvl_w, cm_f, cm_b;
+    DevVec seg_work;                // chunk segment bounds, chunk counts and chunk offsets
     DevVec tile_ord, tile_k, tile_v, tile_tmp;   // SpMV tile order (by the user index of a tile's first row)
     DevVec pos_of_user;             // position of every user-numbered row
     DevVec lf_off, lf_width, lf_ind, lf_val, lb_off, lb_width, lb_ind, lb_val;
@@ -117,7 +118,8 @@ struct rc_handle_s {
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
-    int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets
+    int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets,
+                                    // 3 spatial within sweep-group classes
     // tail of the level order (LDS-resident solve)
     int tail_max_rows = 256, tail_lds_cap = -1;
     int nap_max = 1;                // measured best: poll again after one s_sleep 2

@@ -230,7 +230,7 @@ def test_tail_width_limits_agree_with_oracle(rc, orc, wl, tail, prec, monkeypatc
     assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("order", ["length", "bucket"])
+@pytest.mark.parametrize("order", ["length", "bucket", "groups"])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_level_orders_agree_with_oracle(rc, orc, wl, order, prec, monkeypatch):
     """The row order inside a level (by width; width classes in input order;
