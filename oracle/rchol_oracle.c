@@ -594,15 +594,22 @@ int orc_ic0(int n, const int64_t *colptr, const int *rowind, const double *val, 
         for (int64_t t = up[i]; t < up[i + 1]; ++t) {
             const int j = ui[t];
             double s = uv[t];
-            int64_t p = up[i], q = up[j];
-            while (p < t && q < up[j + 1]) {
-                if (ui[p] == ui[q]) {
-                    s -= mutation == ORC_MUT_IC0_NO_PIVOT ? l[p] * l[q] : l[p] * l[q] * diag[ui[p]];
-                    ++p;
-                    ++q;
+            /* common neighbours m < j in ascending order: each m of row j is
+             * looked up in row i's entries before t by bisection (a dense row,
+             * such as an appended ground vertex, would make a linear merge
+             * quadratic in its length) */
+            int64_t p = up[i];
+            for (int64_t q = up[j]; q < up[j + 1]; ++q) {
+                const int mq = ui[q];
+                int64_t hi = t;
+                while (p < hi) {
+                    const int64_t mid = p + ((hi - p) >> 1);
+                    if (ui[mid] < mq) p = mid + 1; else hi = mid;
                 }
-                else if (ui[p] < ui[q]) ++p;
-                else ++q;
+                if (p < t && ui[p] == mq) {
+                    s -= mutation == ORC_MUT_IC0_NO_PIVOT ? l[p] * l[q] : l[p] * l[q] * diag[mq];
+                    ++p;
+                }
             }
             const double dj = diag[j];
             const double lij = dj > 0.0 ? s / dj : 0.0;

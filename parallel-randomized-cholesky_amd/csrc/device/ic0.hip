@@ -10,11 +10,17 @@
 // preconditioner's D+ rule skips it as it does the randomized factor's.
 //
 // Rows are computed by persistent waves that claim chunks of 64 consecutive
-// rows in increasing order; a row reads the rows it depends on (lower
-// indices) once their pivot is published (pivots start as a sentinel and are
-// stored last, after the row's values and level).  A claimed row's
-// dependencies were claimed earlier by running waves, so the smallest
-// unfinished row can always proceed.  Every wait is bounded.
+// rows in increasing order, one row per lane.  A row reads the rows it
+// depends on (lower indices) once their pivot is published (pivots start as a
+// sentinel and are stored last, after the row's values and level).  Lanes
+// never block: every pass of a wave's loop, each lane whose next dependency
+// is published consumes it, the others poll again on the next pass, so a
+// lane waiting for a row of its own wave never holds that row back (a chunk
+// processed by one lane row after row would chain every chunk to the last
+// rows of the chunks before it).  A claimed row's dependencies were claimed
+// earlier by running waves, so the smallest unfinished row can always
+// proceed.  A wave that makes no progress sleeps briefly between passes; its
+// waiting is bounded.
 #include "common.cuh"
 #include "kernels.h"
 
@@ -24,58 +30,6 @@ namespace {
 
 constexpr int kIcChunk = 64;
 
-// Rows c0 .. c0+63 by one lane; returns 1 when a wait expired.
-__device__ __noinline__ int ic0_chunk(const Ic0Args &a, int c0)
-{
-    const int c1 = min(a.n, c0 + kIcChunk);
-    for (int i = c0; i < c1; ++i) {
-        const int b = a.up_ptr[i], e = a.up_ptr[i + 1];
-        double di = a.diag_a[i];
-        int lev = 0;
-        for (int t = b; t < e; ++t) {
-            const int j = a.up_ind[t];
-            // wait for row j (its pivot is published last)
-            double dj = ld_relaxed_val(a.d + j);
-            if (is_sent(dj)) {
-                const long long t0 = wall_clock64();
-                while (is_sent(dj = ld_relaxed_val(a.d + j))) {
-                    if ((ld_relaxed(a.status) & 1u) || wall_clock64() - t0 > a.spin_ticks) {
-                        atomicOr(a.status, 1u);
-                        return 1;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-            fence_acquire();
-            // s = a_ij - sum over common m < j of l_im l_jm d_m (both rows sorted)
-            double s = a.up_val[t];
-            int p = b, q = a.up_ptr[j];
-            const int qe = a.up_ptr[j + 1];
-            while (p < t && q < qe) {
-                const int mp = a.up_ind[p], mq = a.up_ind[q];
-                if (mp == mq) {
-                    s -= a.l[p] * a.l[q] * a.d[mp];
-                    ++p;
-                    ++q;
-                } else if (mp < mq) {
-                    ++p;
-                } else {
-                    ++q;
-                }
-            }
-            const double lij = dj > 0.0 ? s / dj : 0.0;
-            a.l[t] = lij;
-            di -= lij * lij * dj;
-            lev = max(lev, a.level[j] + 1);
-        }
-        if (!(di > 1e-14 * a.diag_a[i])) di = 0.0;
-        a.level[i] = lev;
-        fence_release();
-        st_relaxed_val(a.d + i, di);
-    }
-    return 0;
-}
-
 __global__ void __launch_bounds__(64) k_ic0(Ic0Args a)
 {
     const int lane = threadIdx.x;
@@ -84,16 +38,83 @@ __global__ void __launch_bounds__(64) k_ic0(Ic0Args a)
         if (lane == 0) c0 = (int)atomicAdd(a.claim, 1u) * kIcChunk;
         c0 = __shfl(c0, 0, 64);
         if (c0 >= a.n) return;
-        int stop = 0;
-        if (lane == 0) stop = ic0_chunk(a, c0);
-        stop = __shfl(stop, 0, 64);
-        if (stop) return;
+        const int i = c0 + lane;
+        bool done = i >= a.n;
+        int b = 0, e = 0, t = 0, lev = 0;
+        double di = 0.0;
+        if (!done && i >= a.n_compute) {
+            // the appended ground row of an SDDM input: trimmed from the
+            // factor, and no row depends on the last one
+            a.level[i] = 0;
+            st_relaxed_val(a.d + i, 0.0);
+            done = true;
+        }
+        if (!done) {
+            b = a.up_ptr[i];
+            e = a.up_ptr[i + 1];
+            t = b;
+            di = a.diag_a[i];
+        }
+        const long long t0 = wall_clock64();
+        while (true) {
+            bool progressed = false;
+            if (!done && t < e) {
+                const int j = a.up_ind[t];
+                const double dj = ld_relaxed_val(a.d + j);
+                if (!is_sent(dj)) {
+                    fence_acquire();
+                    // s = a_ij - sum over common m < j of l_im l_jm d_m, m
+                    // ascending: every m of row j (short) is looked up in
+                    // row i's entries before t by bisection, so a dense row
+                    // costs O(log) per common neighbour, not a walk of its
+                    // whole prefix
+                    double sum = a.up_val[t];
+                    int lo = b;
+                    for (int q = a.up_ptr[j]; q < a.up_ptr[j + 1]; ++q) {
+                        const int mq = a.up_ind[q];
+                        int hi = t;                       // first p in [lo, t) with up_ind[p] >= mq
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (a.up_ind[mid] < mq) lo = mid + 1; else hi = mid;
+                        }
+                        if (lo < t && a.up_ind[lo] == mq) {
+                            sum -= a.l[lo] * a.l[q] * a.d[mq];
+                            ++lo;
+                        }
+                    }
+                    const double lij = dj > 0.0 ? sum / dj : 0.0;
+                    a.l[t] = lij;
+                    di -= lij * lij * dj;
+                    lev = max(lev, a.level[j] + 1);
+                    ++t;
+                    progressed = true;
+                }
+            }
+            if (!done && t >= e) {
+                if (!(di > 1e-14 * a.diag_a[i])) di = 0.0;
+                a.level[i] = lev;
+                fence_release();
+                st_relaxed_val(a.d + i, di);
+                done = true;
+                progressed = true;
+            }
+            if (__all(done)) break;
+            if (!__any(progressed)) {
+                if ((ld_relaxed(a.status) & 1u) || wall_clock64() - t0 > a.spin_ticks) {
+                    if (lane == 0) atomicOr(a.status, 1u);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
     }
 }
 
 // L by columns in the elimination's output layout: row i's entry l_ij goes to
 // the position of row i in column j of the strict-lower slice (rows sorted);
 // out_val holds the entries of L, as the randomized elimination's -w/S do.
+// Rows below n only (n excludes a trimmed ground row, whose entries the
+// assembly drops).
 __global__ void k_ic0_scatter(int n, const int *up_ptr, const int *up_ind, const double *l, const int *lo_ptr,
                               const int *lo_row, double *out_val)
 {

@@ -477,6 +477,7 @@ void launch_gumbel_keys(int n, const float *score, uint64_t seed, uint32_t *hi, 
 // ------------------------------------------------------------ IC(0) (ic0.hip)
 struct Ic0Args {
     int n;
+    int n_compute;                // rows computed: n, or n - 1 when the last (ground) row is trimmed
     const int *up_ptr, *up_ind;   // row i of L: the rows j < i of column i of the prepared matrix
     const double *up_val, *diag_a;
     double *l;                    // values of L in the up_* layout

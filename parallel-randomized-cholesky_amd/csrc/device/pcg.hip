@@ -628,22 +628,26 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         const bool lead = t < stride && i < d.y && g == G - 1;
         const int q = d.x - tp0 + i;
         T acc = (T)0;
+        // a wave with no slot of this level skips its LDS work (its loads of
+        // the padding block were issued all the same, see fetch)
+        if ((t & ~(kWave - 1)) < stride) {
 #pragma unroll
-        for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
-        if (t < stride) {
-            // blocks past the prefetched one (long rows); a padded entry adds
-            // -0 * 0 = -0, which leaves acc unchanged
-            for (int sl = kTailPref; sl < d.w; sl += kTailBlock) {
-                int jj[kTailBlock];
-                T ll[kTailBlock];
-                tail_block_load<T>(ind, val, d.z + sl * stride + t * kTailBlock, jj, ll);
+            for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
+            if (t < stride) {
+                // blocks past the prefetched one (long rows); a padded entry
+                // adds -0 * 0 = -0, which leaves acc unchanged
+                for (int sl = kTailPref; sl < d.w; sl += kTailBlock) {
+                    int jj[kTailBlock];
+                    T ll[kTailBlock];
+                    tail_block_load<T>(ind, val, d.z + sl * stride + t * kTailBlock, jj, ll);
 #pragma unroll
-                for (int u = 0; u < kTailBlock; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
+                    for (int u = 0; u < kTailBlock; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
+                }
             }
+            if (lead) acc += v[q];
+            acc = group_sum(acc, G);
+            if (lead) v[q] = acc;
         }
-        if (lead) acc += v[q];
-        acc = group_sum(acc, G);
-        if (lead) v[q] = acc;
         lds_barrier();
     };
     int4 dA = make_int4(0, 0, 0, 0), dB = dA, dC = dA;

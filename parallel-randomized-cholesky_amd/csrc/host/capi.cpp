@@ -117,6 +117,8 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (sc) h->seg_cap_limit = std::max(1, std::atoi(sc));
         const char *at = std::getenv("RCHOL_ASYNC_TAIL");       // 0: narrow rounds stay round kernels
         if (at) h->async_tail = std::atoi(at) != 0;
+        const char *aw = std::getenv("RCHOL_ASYNC_WIDTH");      // round width at which the queue-driven tail starts
+        if (aw) h->async_width = std::max(1, std::atoi(aw));
         const char *mg = std::getenv("RCHOL_SWEEP_MIN_GROUP");  // lanes per row of the widest sweep chunks (1, 2, 4)
         if (mg) {
             const int v = std::atoi(mg);
@@ -693,7 +695,7 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
             unsigned long long pending = probe[kListSegs];
             for (int q = 0; q < kListSegs; ++q) pending += std::min<unsigned long long>(probe[q], (unsigned long long)a.seg_cap);
             if (st != 0 || pending == 0) break;
-            if (h->async_tail && pending <= (unsigned long long)kBigGrid) {
+            if (h->async_tail && pending <= (unsigned long long)h->async_width) {
                 // narrow from here: the rest runs in one queue-driven launch
                 std::vector<unsigned int> cnt((size_t)round);
                 e = hipMemcpyAsync(cnt.data(), h->round_count.p, cnt.size() * sizeof(unsigned int),
@@ -1246,6 +1248,7 @@ static int rc_factor_ic0_impl(rc_handle_t h, rc_factor_stats *st)
     launch_fill_sentinel<double>(h->diag64.as<double>(), nf, s);
     Ic0Args a;
     a.n = nf;
+    a.n_compute = h->trim ? nf - 1 : nf;
     a.up_ptr = h->ic_up_ptr.as<int>();
     a.up_ind = h->ic_up_ind.as<int>();
     a.up_val = h->ic_up_val.as<double>();
@@ -1258,7 +1261,7 @@ static int rc_factor_ic0_impl(rc_handle_t h, rc_factor_stats *st)
     a.status = cw + 2;
     a.spin_ticks = h->spin_ticks;
     launch_ic0(a, std::max(64, h->num_blocks * 4), s);
-    launch_ic0_scatter(nf, a.up_ptr, a.up_ind, a.l, h->lo_ptr.as<int>(), h->lo_row.as<int>(),
+    launch_ic0_scatter(a.n_compute, a.up_ptr, a.up_ind, a.l, h->lo_ptr.as<int>(), h->lo_row.as<int>(),
                        h->out_val.as<double>(), s);
     RC_HIP(hipGetLastError());
     RC_HIP(hipEventRecord(e1, s));

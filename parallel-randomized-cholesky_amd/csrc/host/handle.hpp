@@ -98,6 +98,7 @@ struct rc_handle_s {
         out_cnt, diag64, scratch;
     DevVec aq, aq_ctr;              // queue of the elimination's queue-driven tail
     bool async_tail = true;         // narrow rounds run as one queue-driven launch
+    int async_width = kBigGrid;     // ... once a round holds at most this many columns
     int sweep_gmin = kSweepMinGroupDefault;   // fewest lanes per row in a sweep chunk (chunks of <= 256 / gmin rows)
     long long pool_cap = 0, out_cap = 0;
     int scratch_cap = 0;
