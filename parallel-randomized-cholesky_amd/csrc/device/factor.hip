@@ -616,8 +616,10 @@ __device__ bool eliminate_column(const FactorArgs &a, int k, int round, char *ld
 }  // namespace
 
 // One elimination round: every column of `cur` (count counts[round]).
-__global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round, const int *cur, int *next)
+__global__ void __launch_bounds__(kWave) k_factor_round(const FactorArgs *__restrict__ pa, int round, const int *cur,
+                                                        int *next)
 {
+    const FactorArgs &a = *pa;
     __shared__ __attribute__((aligned(16))) char lds[kLdsCap * kBytesPerEntry];
     // this round's list counters (one per lane) with the status in one trip;
     // workgroup 0 clears the counter set of round + 2 (last used by round - 1)
@@ -653,8 +655,10 @@ __global__ void __launch_bounds__(kWave) k_factor_round(FactorArgs a, int round,
 // The round's columns deferred by k_factor_round (more than kLdsCap entries):
 // one wave per column with kBigCap entries of LDS (columns above that use the
 // per-workgroup global scratch).  Runs after k_factor_round of the same round.
-__global__ void __launch_bounds__(kWave) k_factor_round_big(FactorArgs a, int round, const int *cur, int *next)
+__global__ void __launch_bounds__(kWave) k_factor_round_big(const FactorArgs *__restrict__ pa, int round, const int *cur,
+                                                            int *next)
 {
+    const FactorArgs &a = *pa;
     extern __shared__ __attribute__((aligned(16))) char lds_big[];
     // a round of at most kBigGrid columns is eliminated here entirely (one
     // column per workgroup); otherwise the columns k_factor_round deferred
@@ -696,8 +700,9 @@ __device__ __forceinline__ bool factor_should_stop(const FactorArgs &a)
     return ld_relaxed(&a.ctr[kCtrStatus]) != 0ull;
 }
 
-__global__ void __launch_bounds__(kWave) k_factor_async(FactorArgs a, unsigned int total)
+__global__ void __launch_bounds__(kWave) k_factor_async(const FactorArgs *__restrict__ pa, unsigned int total)
 {
+    const FactorArgs &a = *pa;
     extern __shared__ __attribute__((aligned(16))) char lds_big[];
     const int lane = lane_id();
     Cursor fcur, ocur;
@@ -802,13 +807,14 @@ void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s
     hipLaunchKernelGGL(k_factor_seed, dim3(1024), dim3(256), 0, s, a, (const int *)a.dep);
 }
 
-void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s)
+void launch_factor_rounds(const FactorArgs &a, const FactorArgs *da, int num_blocks, int first_round, int nrounds,
+                          hipStream_t s)
 {
     for (int r = first_round; r < first_round + nrounds; ++r) {
         const int *cur = (r & 1) ? a.list1 : a.list0;
         int *next = (r & 1) ? a.list0 : a.list1;
-        hipLaunchKernelGGL(k_factor_round, dim3(num_blocks), dim3(kWave), 0, s, a, r, cur, next);
-        hipLaunchKernelGGL(k_factor_round_big, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, a, r,
+        hipLaunchKernelGGL(k_factor_round, dim3(num_blocks), dim3(kWave), 0, s, da, r, cur, next);
+        hipLaunchKernelGGL(k_factor_round_big, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, da, r,
                            cur, next);
     }
 }
@@ -821,14 +827,14 @@ void factor_kernel_setup()
                               (int)((size_t)kBigCap * kBytesPerEntry));
 }
 
-void launch_factor_async(const FactorArgs &a0, int round, unsigned int total, hipStream_t s)
+void launch_factor_async(const FactorArgs &a0, const FactorArgs *da, int round, unsigned int total, hipStream_t s)
 {
     FactorArgs a = a0;
     a.aq_total = total;            // total <= n - 1 slots of the n-entry queue
     const int *cur = (round & 1) ? a.list1 : a.list0;
     // one block: the init kernel's slot clearing must finish before it seeds the queue
     hipLaunchKernelGGL(k_factor_async_init, dim3(1), dim3(1024), 0, s, a, round, cur, total);
-    hipLaunchKernelGGL(k_factor_async, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, a, total);
+    hipLaunchKernelGGL(k_factor_async, dim3(kBigGrid), dim3(kWave), (size_t)kBigCap * kBytesPerEntry, s, da, total);
 }
 
 int factor_lds_capacity() { return kLdsCap; }

@@ -94,11 +94,16 @@ struct FactorArgs {
 };
 
 void launch_factor_setup(const FactorArgs &a, const int *dep_init, hipStream_t s);
-void launch_factor_rounds(const FactorArgs &a, int num_blocks, int first_round, int nrounds, hipStream_t s);
+// da: a device copy of a (the round and queue kernels read their arguments
+// through it, so they are loaded when used instead of held in scalar
+// registers for the whole kernel)
+void launch_factor_rounds(const FactorArgs &a, const FactorArgs *da, int num_blocks, int first_round, int nrounds,
+                          hipStream_t s);
 void launch_factor_finish(const FactorArgs &a, hipStream_t s);
 // The rest of the elimination from round `round` (whose list holds at most
 // kBigGrid columns) in one persistent launch; `total` columns remain.
-void launch_factor_async(const FactorArgs &a, int round, unsigned int total, hipStream_t s);
+// da->aq_total must already hold `total` (the caller writes it into the device copy)
+void launch_factor_async(const FactorArgs &a, const FactorArgs *da, int round, unsigned int total, hipStream_t s);
 int factor_lds_capacity();
 int factor_scratch_bytes_per_entry();
 long long factor_pool_slack(int num_blocks);

@@ -663,6 +663,10 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         int rc = alloc_factor_work(h);
         if (rc != RC_OK) return rc;
         FactorArgs a = factor_args(h);
+        // the round and queue kernels read their arguments from a device copy
+        RC_HIP(h->fargs.ensure(sizeof(FactorArgs)));
+        RC_HIP(hipMemcpyAsync(h->fargs.p, &a, sizeof(FactorArgs), hipMemcpyHostToDevice, s));
+        const FactorArgs *da = h->fargs.as<FactorArgs>();
         hipEvent_t e0, e1;
         RC_HIP(hipEventCreate(&e0));
         RC_HIP(hipEventCreate(&e1));
@@ -683,7 +687,7 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
         while (true) {
             int nr = std::min(batch, h->nf + 1 - round);
             if (nr <= 0) break;
-            launch_factor_rounds(a, h->num_blocks, round, nr, s);
+            launch_factor_rounds(a, da, h->num_blocks, round, nr, s);
             round += nr;
             hipError_t e = hipMemcpyAsync(probe, h->seg_count.as<unsigned int>() + (size_t)(round % 3) * kListSlots,
                                           kListSlots * sizeof(unsigned int), hipMemcpyDeviceToHost, s);
@@ -706,7 +710,11 @@ static int run_elimination(rc_handle_t h, double *ms_out, int *regrowths)
                 for (unsigned int v : cnt) before += v;
                 async_total = (unsigned long long)(h->nf - 1) - before;
                 async_round = round;
-                launch_factor_async(a, round, (unsigned int)async_total, s);
+                const unsigned int aqt = (unsigned int)async_total;
+                e = hipMemcpyAsync((char *)h->fargs.p + offsetof(FactorArgs, aq_total), &aqt, sizeof(aqt),
+                                   hipMemcpyHostToDevice, s);
+                if (e != hipSuccess) { (void)hipHostFree(probe); return fail(h, RC_ERR_HIP, hipGetErrorString(e)); }
+                launch_factor_async(a, da, round, aqt, s);
                 break;
             }
             batch = std::min(batch * 2, 64);     // bounds the empty rounds launched past the end

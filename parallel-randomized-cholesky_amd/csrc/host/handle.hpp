@@ -97,6 +97,7 @@ struct rc_handle_s {
     DevVec dep, head, level, list0, list1, seg_count, round_count, wave_state, ctr, pool, out_row, out_val, out_start,
         out_cnt, diag64, scratch;
     DevVec aq, aq_ctr;              // queue of the elimination's queue-driven tail
+    DevVec fargs;                   // device copy of the elimination's kernel arguments
     bool async_tail = true;         // narrow rounds run as one queue-driven launch
     int async_width = kBigGrid;     // ... once a round holds at most this many columns
     int sweep_gmin = kSweepMinGroupDefault;   // fewest lanes per row in a sweep chunk (chunks of <= 256 / gmin rows)
