@@ -84,25 +84,50 @@ __global__ void __launch_bounds__(kMB) k_mspmv(MultiArgs a, const double *p)
         r0 = lo + (blockIdx.x / 8) * rows_per_block + threadIdx.x / K;
         rstep = (gridDim.x / 8) * rows_per_block;
     }
-    for (int ru = r0; ru < r1; ru += rstep) {
-        const int row = a.row_ord[ru];
-        double acc = 0.0;
-        const int e1 = a.a_ptr[row + 1];
-        int e = a.a_ptr[row];
-        for (; e + 4 <= e1; e += 4) {
-            int j[4];
-            double v[4], pj[4];
+    // two rows per pass, the first kE entries of both loaded together, so the
+    // dependent loads of one row (row index, row bounds, entries, gathers)
+    // overlap with the other's; each row still sums its entries in order
+    constexpr int kE = 8;
+    for (int ru = r0; ru < r1; ru += 2 * rstep) {
+        const bool two = ru + rstep < r1;
+        int row[2], eb[2], ee[2];
+        row[0] = a.row_ord[ru];
+        row[1] = two ? a.row_ord[ru + rstep] : row[0];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { j[u] = a.a_ind[e + u]; v[u] = a.a_val[e + u]; }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = p[(size_t)j[u] * K + q];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc = fma_t(v[u], pj[u], acc);
+        for (int h = 0; h < 2; ++h) {
+            eb[h] = a.a_ptr[row[h]];
+            ee[h] = a.a_ptr[row[h] + 1];
         }
-        for (; e < e1; ++e) acc = fma_t(a.a_val[e], p[(size_t)a.a_ind[e] * K + q], acc);
-        const size_t o = (size_t)row * K + q;
-        a.ap[o] = acc;
-        acc_dot += p[o] * acc;
+        if (!two) ee[1] = eb[1];
+        int j[2][kE];
+        double v[2][kE], pj[2][kE];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < kE; ++u) {
+                const bool has = eb[h] + u < ee[h];
+                j[h][u] = has ? a.a_ind[eb[h] + u] : 0;
+                v[h][u] = has ? a.a_val[eb[h] + u] : 0.0;
+            }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < kE; ++u) pj[h][u] = p[(size_t)j[h][u] * K + q];
+        double acc[2] = {0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int u = 0; u < kE; ++u)
+                if (eb[h] + u < ee[h]) acc[h] = fma_t(v[h][u], pj[h][u], acc[h]);
+            for (int e = eb[h] + kE; e < ee[h]; ++e) acc[h] = fma_t(a.a_val[e], p[(size_t)a.a_ind[e] * K + q], acc[h]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !two) break;
+            const size_t o = (size_t)row[h] * K + q;
+            a.ap[o] = acc[h];
+            acc_dot += p[o] * acc[h];
+        }
     }
     // per-q block partial: threads t, t+K, t+2K, ... share q
     red[threadIdx.x] = acc_dot;
