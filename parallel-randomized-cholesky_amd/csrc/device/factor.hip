@@ -39,24 +39,32 @@ namespace {
 
 constexpr int kLdsCap = 128;                                   // entries per wave in LDS (round kernel)
 static_assert(kLdsCap <= 2 * 64, "the round kernel sorts its columns in registers (two slots per lane)");
-constexpr int kBytesPerEntry = 8 + 8 + 8 + 4 + 4 + 4 + 4 + 4 + 4;  // sw mw C srow smult mrow mmult minc jt
+constexpr int kBytesPerEntry = 16 + 8 + 8 + 4 + 4 + 4 + 4;  // s mw C mrow mmult minc jt
 constexpr long long kFillChunk = 2048;
 constexpr long long kOutChunk = 2048;
 
+// An entry being sorted: row, payload (multiplicity, later the merged index)
+// and weight in one 16-byte record, so a comparator moves two records with
+// one LDS instruction each instead of six.
+struct __attribute__((aligned(16))) SEnt {
+    int row;
+    int pay;
+    double w;
+};
+
 struct ColBuf {
-    double *sw, *mw, *C;
-    int *srow, *smult, *mrow, *mmult, *minc, *jt;   // jt: sampled partner of every fill
+    SEnt *s;
+    double *mw, *C;
+    int *mrow, *mmult, *minc, *jt;   // jt: sampled partner of every fill
 };
 
 __device__ __forceinline__ ColBuf carve(char *base, int cap)
 {
     ColBuf b;
-    b.sw = reinterpret_cast<double *>(base);
-    b.mw = b.sw + cap;
+    b.s = reinterpret_cast<SEnt *>(base);
+    b.mw = reinterpret_cast<double *>(b.s + cap);
     b.C = b.mw + cap;
-    b.srow = reinterpret_cast<int *>(b.C + cap);
-    b.smult = b.srow + cap;
-    b.mrow = b.smult + cap;
+    b.mrow = reinterpret_cast<int *>(b.C + cap);
     b.mmult = b.mrow + cap;
     b.minc = b.mmult + cap;
     b.jt = b.minc + cap;
@@ -73,21 +81,27 @@ __device__ __forceinline__ int next_pow2(int v)
 // Ascending sorting network (bitonic with a mirrored first comparator in
 // every merge stage); every comparator puts the minimum at the lower index, so
 // positions >= len behave as +infinity and need no padding.
-// kByRow: key (row, w); otherwise key (w, row).  pay is carried if non-null.
+// kByRow: key (row, w); otherwise key (w, row); the payload travels along.
 // The comparators of a stage touch disjoint pairs, so each lane reads all of
 // its pairs of the stage (kNB = P / 128 of them, at most 8 at a time) before
 // it compares and writes any: one LDS round trip per stage instead of one per
 // 64 comparators.  kNB is a template argument so no slot is computed idly.
+template <bool kByRow>
+__device__ __forceinline__ bool key_less(int ra, double wa, int rb, double wb)
+{
+    return kByRow ? (ra < rb || (ra == rb && wa < wb)) : (wa < wb || (wa == wb && ra < rb));
+}
+
 template <bool kByRow, int kNB>
-__device__ __forceinline__ void wave_sort_nb(int *row, double *w, int *pay, int len, int P)
+__device__ __forceinline__ void wave_sort_nb(SEnt *e, int len, int P)
 {
     const int lane = lane_id();
     const int half = P >> 1;
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t0 = 0; t0 < half; t0 += kNB * kWave) {
-                int ia[kNB], la[kNB], ri[kNB], rl[kNB], pi[kNB], pl[kNB];
-                double wi[kNB], wl[kNB];
+                int ia[kNB], la[kNB];
+                SEnt ei[kNB], el[kNB];
                 bool act[kNB];
 #pragma unroll
                 for (int b = 0; b < kNB; ++b) {
@@ -99,21 +113,14 @@ __device__ __forceinline__ void wave_sort_nb(int *row, double *w, int *pay, int 
                     la[b] = l;
                     // read unconditionally (i, l < P <= the buffer's capacity),
                     // so no branch sits between the reads of the slots
-                    ri[b] = row[i];
-                    rl[b] = row[l];
-                    wi[b] = w[i];
-                    wl[b] = w[l];
-                    pi[b] = pay ? pay[i] : 0;
-                    pl[b] = pay ? pay[l] : 0;
+                    ei[b] = e[i];
+                    el[b] = e[l];
                 }
 #pragma unroll
                 for (int b = 0; b < kNB; ++b) {
-                    const bool gt = kByRow ? (ri[b] > rl[b] || (ri[b] == rl[b] && wi[b] > wl[b]))
-                                           : (wi[b] > wl[b] || (wi[b] == wl[b] && ri[b] > rl[b]));
-                    if (act[b] && gt) {
-                        row[ia[b]] = rl[b]; row[la[b]] = ri[b];
-                        w[ia[b]] = wl[b]; w[la[b]] = wi[b];
-                        if (pay) { pay[ia[b]] = pl[b]; pay[la[b]] = pi[b]; }
+                    if (act[b] && key_less<kByRow>(el[b].row, el[b].w, ei[b].row, ei[b].w)) {
+                        e[ia[b]] = el[b];
+                        e[la[b]] = ei[b];
                     }
                 }
             }
@@ -123,12 +130,12 @@ __device__ __forceinline__ void wave_sort_nb(int *row, double *w, int *pay, int 
 }
 // Networks of P >= 256 positions (shorter columns use the register sorts).
 template <bool kByRow>
-__device__ __forceinline__ void wave_sort(int *row, double *w, int *pay, int len)
+__device__ __forceinline__ void wave_sort(SEnt *e, int len)
 {
     const int P = next_pow2(len);
-    if (P <= 256) wave_sort_nb<kByRow, 2>(row, w, pay, len, 256);
-    else if (P <= 512) wave_sort_nb<kByRow, 4>(row, w, pay, len, 512);
-    else wave_sort_nb<kByRow, 8>(row, w, pay, len, P);
+    if (P <= 256) wave_sort_nb<kByRow, 2>(e, len, 256);
+    else if (P <= 512) wave_sort_nb<kByRow, 4>(e, len, 512);
+    else wave_sort_nb<kByRow, 8>(e, len, P);
 }
 
 // The same ordering for len <= 64 R entries held in registers, entry i in
@@ -137,14 +144,8 @@ __device__ __forceinline__ void wave_sort(int *row, double *w, int *pay, int len
 // above, with no LDS round trips between stages.  Positions >= len hold
 // +infinity keys.  Ties keep their values (the sorted output is unique: equal
 // keys carry equal payloads in both uses).
-template <bool kByRow>
-__device__ __forceinline__ bool key_less(int ra, double wa, int rb, double wb)
-{
-    return kByRow ? (ra < rb || (ra == rb && wa < wb)) : (wa < wb || (wa == wb && ra < rb));
-}
-
 template <bool kByRow, int R>
-__device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len)
+__device__ __forceinline__ void lane_sort(SEnt *e, int len)
 {
     const int lane = lane_id();
     const int P = next_pow2(len);
@@ -153,9 +154,14 @@ __device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len
 #pragma unroll
     for (int s = 0; s < R; ++s) {
         const int i = s * kWave + lane;
-        r[s] = i < len ? row[i] : 0x7fffffff;
-        x[s] = i < len ? w[i] : __builtin_inf();
-        y[s] = (i < len && pay) ? pay[i] : 0;
+        SEnt v;
+        v.row = 0x7fffffff;
+        v.pay = 0;
+        v.w = __builtin_inf();
+        if (i < len) v = e[i];
+        r[s] = v.row;
+        x[s] = v.w;
+        y[s] = v.pay;
     }
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -194,9 +200,11 @@ __device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len
     for (int s = 0; s < R; ++s) {
         const int i = s * kWave + lane;
         if (i < len) {
-            row[i] = r[s];
-            w[i] = x[s];
-            if (pay) pay[i] = y[s];
+            SEnt v;
+            v.row = r[s];
+            v.pay = y[s];
+            v.w = x[s];
+            e[i] = v;
         }
     }
     __syncthreads();
@@ -209,18 +217,272 @@ __device__ __forceinline__ void lane_sort(int *row, double *w, int *pay, int len
 // longer columns), so the LDS network is not compiled in (its registers would
 // lower the round kernel's occupancy).
 template <bool kByRow, int kMaxR, bool kShort = false>
-__device__ __forceinline__ void column_sort(int *row, double *w, int *pay, int len)
+__device__ __forceinline__ void column_sort(SEnt *e, int len)
 {
     if constexpr (kShort) {
-        if (len <= kWave) lane_sort<kByRow, 1>(row, w, pay, len);
-        else lane_sort<kByRow, kMaxR>(row, w, pay, len);
+        if (len <= kWave) lane_sort<kByRow, 1>(e, len);
+        else lane_sort<kByRow, kMaxR>(e, len);
         return;
     }
-    if (len <= kWave) lane_sort<kByRow, 1>(row, w, pay, len);
-    else if (kMaxR >= 2 && len <= 2 * kWave) lane_sort<kByRow, (kMaxR >= 2 ? 2 : 1)>(row, w, pay, len);
-    else if (kMaxR >= 4 && len <= 4 * kWave) lane_sort<kByRow, (kMaxR >= 4 ? 4 : 1)>(row, w, pay, len);
-    else if (kMaxR >= 8 && len <= 8 * kWave) lane_sort<kByRow, (kMaxR >= 8 ? 8 : 1)>(row, w, pay, len);
-    else wave_sort<kByRow>(row, w, pay, len);
+    if (len <= kWave) lane_sort<kByRow, 1>(e, len);
+    else if (kMaxR >= 2 && len <= 2 * kWave) lane_sort<kByRow, (kMaxR >= 2 ? 2 : 1)>(e, len);
+    else if (kMaxR >= 4 && len <= 4 * kWave) lane_sort<kByRow, (kMaxR >= 4 ? 4 : 1)>(e, len);
+    else if (kMaxR >= 8 && len <= 8 * kWave) lane_sort<kByRow, (kMaxR >= 8 ? 8 : 1)>(e, len);
+    else wave_sort<kByRow>(e, len);
+}
+
+// ---- counting sorts for the kernels that run one wave per CU
+// A column of at most kRankMax entries is put in order by counting, for every
+// entry, the keys below its own: each lane compares its kRankSlots-or-fewer
+// keys with every key of the column (broadcast LDS reads, kRankBatch keys in
+// flight at a time) and then stores each entry at its count: one LDS round
+// trip per kRankBatch keys, where the network needs one per stage (21-36
+// stages).  The large-LDS kernels run one wave per CU, so the VALU is
+// theirs; the round kernel keeps its register networks (its six waves per
+// SIMD share the VALU).  Above two keys per lane the comparisons cost more
+// than the network's round trips (measured: 129-256 entries 11.3 -> 13.5 us
+// for the row order, 9.7 -> 12.9 us for the value order).
+constexpr int kRankMax = 128;
+constexpr int kRankSlots = kRankMax / kWave;
+static_assert(kRankSlots == 2, "the dispatchers take one or two keys per lane");
+constexpr int kRun = 8;    // duplicates of one row merged in registers; a longer run uses the network
+
+// ascending 8-input sorting network (19 comparators in 6 layers)
+__device__ __forceinline__ void sort8(double *v)
+{
+    auto cx = [&](int x, int y) {
+        const double lo = fmin(v[x], v[y]);
+        const double hi = fmax(v[x], v[y]);
+        v[x] = lo;
+        v[y] = hi;
+    };
+    cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+    cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+    cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+    cx(2, 4); cx(3, 5);
+    cx(1, 4); cx(3, 6);
+    cx(1, 2); cx(3, 4); cx(5, 6);
+}
+
+// Entries [0, m) of b.s reordered by (row, gathered index): the
+// key (row << 32 | index) is unique, so the counts are the positions.  The
+// keys are staged in b.C (unused until the scan), padded with +infinity to a
+// multiple of kRankBatch; every batch of keys is read before any is compared,
+// so a batch costs one LDS round trip.
+constexpr int kRankBatch = 16;
+__device__ __forceinline__ int rank_pad(int m) { return (m + kRankBatch - 1) & ~(kRankBatch - 1); }
+
+template <int R>
+__device__ __forceinline__ void rank_by_row(const ColBuf &b, int m)
+{
+    const int lane = lane_id();
+    unsigned long long *key = reinterpret_cast<unsigned long long *>(b.C);
+    int r[R], mu[R], rk[R];
+    double w[R];
+    unsigned long long k[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int i = s * kWave + lane;
+        const bool in = i < m;
+        SEnt v;
+        v.row = 0;
+        v.pay = 0;
+        v.w = 0.0;
+        if (in) v = b.s[i];
+        r[s] = v.row;
+        w[s] = v.w;
+        mu[s] = v.pay;
+        k[s] = in ? (((unsigned long long)(unsigned int)r[s] << 32) | (unsigned int)i) : ~0ull;
+        if (in) key[i] = k[s];
+        rk[s] = 0;
+    }
+    const int mp = rank_pad(m);
+    if (m + lane < mp) key[m + lane] = ~0ull;
+    __syncthreads();
+    const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(key);
+    for (int o = 0; o < mp; o += kRankBatch) {
+        ulonglong2 kk[kRankBatch / 2];
+#pragma unroll
+        for (int q = 0; q < kRankBatch / 2; ++q) kk[q] = kp[o / 2 + q];
+#pragma unroll
+        for (int q = 0; q < kRankBatch / 2; ++q) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) rk[s] += (kk[q].x < k[s] ? 1 : 0) + (kk[q].y < k[s] ? 1 : 0);
+        }
+    }
+    __syncthreads();   // every entry is in registers: the arrays are rewritten in place
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        if (s * kWave + lane < m) {
+            SEnt v;
+            v.row = r[s];
+            v.pay = mu[s];
+            v.w = w[s];
+            b.s[rk[s]] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// Step 2 by counting: entries ordered by (row, index), then every run of one
+// row (at most kRun entries) is put in ascending weight in registers and
+// summed left to right -- the sums of the (row, weight) sort and merge, bit
+// for bit (equal weights are interchangeable).  Returns the merged count, or
+// -1 if some run is longer than kRun (the caller then sorts by network).
+__device__ __forceinline__ int rank_merge(const ColBuf &b, int m_in)
+{
+    const int lane = lane_id();
+    const int m = __builtin_amdgcn_readfirstlane(m_in);
+    const int R = (m + kWave - 1) / kWave;
+    if (R <= 1) rank_by_row<1>(b, m);
+    else rank_by_row<kRankSlots>(b, m);
+    int d = 0;
+    bool over = false;
+    for (int base_i = 0; base_i < m; base_i += kWave) {
+        const int i = base_i + lane;
+        bool head = false;
+        int rr = 0, ms = 0;
+        double ws = 0.0;
+        if (i < m) {
+            rr = b.s[i].row;
+            head = (i == 0) || (b.s[i - 1].row != rr);
+        }
+        if (head) {
+            // the run is [i, i + g): rows are in order; reads past m stay
+            // inside the buffer (m + kRun <= its capacity) and are masked
+            double v[kRun];
+            int g = 0;
+#pragma unroll
+            for (int t = 0; t < kRun; ++t) {
+                const int e = i + t;
+                const SEnt x = b.s[e];
+                const bool in = e < m && x.row == rr;
+                v[t] = in ? x.w : __builtin_inf();
+                ms += in ? x.pay : 0;
+                g += in ? 1 : 0;
+            }
+            over = over || (i + kRun < m && b.s[i + kRun].row == rr);
+            sort8(v);
+            ws = v[0];
+#pragma unroll
+            for (int t = 1; t < kRun; ++t)
+                if (t < g) ws += v[t];
+        }
+        const uint64_t mask = ballot(head);
+        if (head) {
+            const int pos = d + lanes_below(mask);
+            b.mrow[pos] = rr;
+            b.mw[pos] = ws;
+            b.mmult[pos] = ms;
+            b.minc[pos] = 0;
+        }
+        d += __popcll(mask);
+    }
+    const bool any_over = ballot(over) != 0ull;
+    __syncthreads();
+    return any_over ? -1 : d;
+}
+
+// Step 3's order by counting: merged entries [0, d) by (w, row) (rows are
+// distinct, so the order is strict), written to b.s (payload = merged
+// index).  mw / mrow must be padded with +infinity / INT_MAX up to
+// rank_pad(d).
+template <int R>
+__device__ __forceinline__ void rank_by_value(const ColBuf &b, int d)
+{
+    const int lane = lane_id();
+    int r[R], rk[R];
+    double w[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int i = s * kWave + lane;
+        r[s] = i < d ? b.mrow[i] : 0x7fffffff;
+        w[s] = i < d ? b.mw[i] : __builtin_inf();
+        rk[s] = 0;
+    }
+    const int dp = rank_pad(d);
+    const double2 *wp = reinterpret_cast<const double2 *>(b.mw);
+    const int2 *rp = reinterpret_cast<const int2 *>(b.mrow);
+    for (int o = 0; o < dp; o += kRankBatch) {
+        double2 wo[kRankBatch / 2];
+        int2 ro[kRankBatch / 2];
+#pragma unroll
+        for (int q = 0; q < kRankBatch / 2; ++q) {
+            wo[q] = wp[o / 2 + q];
+            ro[q] = rp[o / 2 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < kRankBatch / 2; ++q) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                rk[s] += (wo[q].x < w[s] || (wo[q].x == w[s] && ro[q].x < r[s])) ? 1 : 0;
+                rk[s] += (wo[q].y < w[s] || (wo[q].y == w[s] && ro[q].y < r[s])) ? 1 : 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int i = s * kWave + lane;
+        if (i < d) {
+            SEnt v;
+            v.row = r[s];
+            v.pay = i;
+            v.w = w[s];
+            b.s[rk[s]] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// Step 2 by the sorting networks: (row, w) order, then each run of one row
+// summed left to right (ascending weight).  Returns the merged count.
+template <int kMaxR, bool kShort>
+__device__ __forceinline__ int sort_merge(const ColBuf &b, int m)
+{
+    const int lane = lane_id();
+    column_sort<true, kMaxR, kShort>(b.s, m);
+    int d = 0;
+    for (int base_i = 0; base_i < m; base_i += kWave) {
+        int i = base_i + lane;
+        bool head = false;
+        int r = 0, ms = 0;
+        double ws = 0.0;
+        if (i < m) {
+            r = b.s[i].row;
+            head = (i == 0) || (b.s[i - 1].row != r);
+            if (head) {
+                // the run [i, i + g) summed left to right, kRun records per
+                // LDS round trip (records past m are masked)
+                ws = b.s[i].w;
+                ms = b.s[i].pay;
+                bool more = true;
+                for (int t0 = i + 1; more; t0 += kRun) {
+                    SEnt x[kRun];
+#pragma unroll
+                    for (int u = 0; u < kRun; ++u) x[u] = b.s[t0 + u < m ? t0 + u : m - 1];
+#pragma unroll
+                    for (int u = 0; u < kRun; ++u) {
+                        more = more && t0 + u < m && x[u].row == r;
+                        if (more) {
+                            ws += x[u].w;
+                            ms += x[u].pay;
+                        }
+                    }
+                }
+            }
+        }
+        uint64_t mask = ballot(head);
+        if (head) {
+            int pos = d + lanes_below(mask);
+            b.mrow[pos] = r;
+            b.mw[pos] = ws;
+            b.mmult[pos] = ms;
+            b.minc[pos] = 0;
+        }
+        d += __popcll(mask);
+    }
+    __syncthreads();
+    return d;
 }
 
 __device__ __forceinline__ void raise_status(unsigned long long *ctr, unsigned long long bits)
@@ -315,9 +577,11 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
 {
     const int lane = lane_id();
     for (int t = lane; t < norig && t < cap; t += kWave) {
-        b.srow[t] = a.lo_row[lo0 + t];
-        b.sw[t] = a.lo_w[lo0 + t];
-        b.smult[t] = 1;
+        SEnt v;
+        v.row = a.lo_row[lo0 + t];
+        v.pay = 1;
+        v.w = a.lo_w[lo0 + t];
+        b.s[t] = v;
     }
     int t = norig;
     while (true) {
@@ -329,9 +593,11 @@ __device__ __forceinline__ int gather_column(const FactorArgs &a, int lo0, int n
             const FillRec rec = a.pool[f];
             const int pos = t + lanes_below(mask);
             if (pos < cap) {
-                b.srow[pos] = rec.row;
-                b.sw[pos] = rec.w;
-                b.smult[pos] = 1;
+                SEnt v;
+                v.row = rec.row;
+                v.pay = 1;
+                v.w = rec.w;
+                b.s[pos] = v;
             }
             f = rec.next;
         }
@@ -351,46 +617,40 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
     const int lane = lane_id();
     const int last = a.n - 1;
     // ---- 2. sort by (row, w); merge duplicates left to right (:520-560)
-    column_sort<true, kMaxR, kShort>(b.srow, b.sw, b.smult, m);
-    int d = 0;
-    for (int base_i = 0; base_i < m; base_i += kWave) {
-        int i = base_i + lane;
-        bool head = false;
-        int r = 0, ms = 0;
-        double ws = 0.0;
-        if (i < m) {
-            r = b.srow[i];
-            head = (i == 0) || (b.srow[i - 1] != r);
-            if (head) {
-                ws = b.sw[i];
-                ms = b.smult[i];
-                for (int t = i + 1; t < m && b.srow[t] == r; ++t) {
-                    ws += b.sw[t];
-                    ms += b.smult[t];
-                }
-            }
-        }
-        uint64_t mask = ballot(head);
-        if (head) {
-            int pos = d + lanes_below(mask);
-            b.mrow[pos] = r;
-            b.mw[pos] = ws;
-            b.mmult[pos] = ms;
-            b.minc[pos] = 0;
-        }
-        d += __popcll(mask);
+    int d = -1;
+    if constexpr (!kShort) {
+        if (m <= kRankMax) d = rank_merge(b, m);
     }
-    __syncthreads();
+    if (d < 0) d = sort_merge<kMaxR, kShort>(b, m);
 
     // ---- 3. sort merged entries by (w, row), carrying the merged index;
     //         forward cumulative sums (:573-593)
-    for (int t = lane; t < d; t += kWave) {
-        b.srow[t] = b.mrow[t];
-        b.sw[t] = b.mw[t];
-        b.smult[t] = t;
+    bool ranked = false;
+    if constexpr (!kShort) {
+        if (d <= kRankMax) {
+            d = __builtin_amdgcn_readfirstlane(d);
+            if (d + lane < rank_pad(d)) {
+                b.mw[d + lane] = __builtin_inf();
+                b.mrow[d + lane] = 0x7fffffff;
+            }
+            __syncthreads();
+            const int R = (d + kWave - 1) / kWave;
+            if (R <= 1) rank_by_value<1>(b, d);
+            else rank_by_value<kRankSlots>(b, d);
+            ranked = true;
+        }
     }
-    __syncthreads();
-    column_sort<false, kMaxR, kShort>(b.srow, b.sw, b.smult, d);
+    if (!ranked) {
+        for (int t = lane; t < d; t += kWave) {
+            SEnt v;
+            v.row = b.mrow[t];
+            v.pay = t;
+            v.w = b.mw[t];
+            b.s[t] = v;
+        }
+        __syncthreads();
+        column_sort<false, kMaxR, kShort>(b.s, d);
+    }
     // canonical blocked scan (shared with the oracle): per 64-entry block the
     // doubling scan x_i += x_{i-off}, off = 1..32, then the previous block's
     // last sum is added
@@ -398,7 +658,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
         double carry = 0.0;
         for (int base_i = 0; base_i < d; base_i += kWave) {
             const int i = base_i + lane;
-            double x = i < d ? b.sw[i] : 0.0;
+            double x = i < d ? b.s[i].w : 0.0;
             for (int off = 1; off < kWave; off <<= 1) {
                 const double o = __shfl_up(x, off, kWave);
                 if (lane >= off) x = x + o;
@@ -425,8 +685,8 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
             if (b.C[mid] - Ci < tgt) lo = mid + 1; else hi = mid;
         }
         b.jt[i] = lo;
-        const int ri = b.srow[i], rj = b.srow[lo];
-        atomicAdd(&b.minc[ri > rj ? b.smult[i] : b.smult[lo]], 1);   // the larger row gains a dependency
+        const SEnt ei = b.s[i], el = b.s[lo];
+        atomicAdd(&b.minc[ei.row > el.row ? ei.pay : el.pay], 1);   // the larger row gains a dependency
     }
     long long fbase = 0, obase = 0;
     if (nfill_out > 0 && !carve_pool(fcur, nfill_out, kFillChunk, &a.ctr[kCtrFillTop], a.pool_cap, fbase)) {
@@ -473,7 +733,7 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
             nd[u] = 1;
             if (e < nfill_out) {
                 const int j = b.jt[e];
-                const int ri = b.srow[e], rj = b.srow[j];
+                const int ri = b.s[e].row, rj = b.s[j].row;
                 const int cc = ri > rj ? rj : ri;
                 const int rec = (int)(fbase + e);
                 old[u] = atomicExch(&a.head[(size_t)cc * kFillLists + (rec & (kFillLists - 1))], rec);
@@ -496,11 +756,12 @@ __device__ __forceinline__ bool process_column(const FactorArgs &a, int k, int r
             }
             if (e < nfill_out) {
                 const int j = b.jt[e];
-                const int ri = b.srow[e], rj = b.srow[j];
+                const SEnt se = b.s[e];
+                const int ri = se.row, rj = b.s[j].row;
                 FillRec fr;
                 fr.row = ri > rj ? ri : rj;
                 fr.next = old[u];
-                fr.w = (b.sw[e] * (S - b.C[e])) / S;
+                fr.w = (se.w * (S - b.C[e])) / S;
                 a.pool[(int)(fbase + e)] = fr;
             }
         }

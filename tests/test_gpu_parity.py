@@ -37,6 +37,18 @@ def _case(wl, name):
         return wl.grid_sddm((7, 7, 7)), "sddm"
     if name == "two_comp":
         return read_mtx_py(golden_path("two_components.mtx")), "laplacian"
+    if name in ("complete_300", "er_600"):
+        # dense graphs: late columns gather many fills of the same row (runs
+        # of duplicates longer than the in-register merge takes) and
+        # columns of 65-512 entries
+        rng = np.random.default_rng(5)
+        n = 300 if name == "complete_300" else 600
+        iu, ju = np.triu_indices(n, 1)
+        if name == "er_600":
+            keep = rng.random(len(iu)) < 0.08
+            iu, ju = iu[keep], ju[keep]
+        w = rng.uniform(0.5, 2.0, len(iu))
+        return wl.laplacian_triplets(n, iu, ju, w), "laplacian"
     if name.startswith("star_"):
         # centre eliminated first: one column with d entries.  300 uses the
         # large-LDS kernel, 3000 its global scratch, 9000 a scratch regrowth
@@ -48,7 +60,7 @@ def _case(wl, name):
 
 
 CASES = ["grid2d_16_log", "grid3d_8", "grid3d_20_log", "rgg_3000", "sddm_7", "two_comp", "star_300", "star_3000",
-         "star_9000"]
+         "star_9000", "complete_300", "er_600"]
 
 
 def _solver(rc, wl, name, prec=64, seed=11, **kw):
