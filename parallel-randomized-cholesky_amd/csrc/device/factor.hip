@@ -213,9 +213,10 @@ __device__ __forceinline__ void lane_sort(SEnt *e, int len)
 // kMaxR bounds the register slots (and so the kernel's register budget).  Both
 // kernels use 2: above 128 entries the LDS network measured faster than
 // 4 or 8 slots per lane.
-// kShort: the caller guarantees len <= kMaxR * 64 (the round kernel defers
-// longer columns), so the LDS network is not compiled in (its registers would
-// lower the round kernel's occupancy).
+// kShort (the round kernel): the caller guarantees len <= kMaxR * 64 (longer
+// columns are deferred), so only the register networks are compiled in (the
+// LDS network's registers would lower the round kernel's occupancy).
+// Otherwise (the large-LDS kernels) only the LDS network is.
 template <bool kByRow, int kMaxR, bool kShort = false>
 __device__ __forceinline__ void column_sort(SEnt *e, int len)
 {
@@ -224,11 +225,11 @@ __device__ __forceinline__ void column_sort(SEnt *e, int len)
         else lane_sort<kByRow, kMaxR>(e, len);
         return;
     }
-    if (len <= kWave) lane_sort<kByRow, 1>(e, len);
-    else if (kMaxR >= 2 && len <= 2 * kWave) lane_sort<kByRow, (kMaxR >= 2 ? 2 : 1)>(e, len);
-    else if (kMaxR >= 4 && len <= 4 * kWave) lane_sort<kByRow, (kMaxR >= 4 ? 4 : 1)>(e, len);
-    else if (kMaxR >= 8 && len <= 8 * kWave) lane_sort<kByRow, (kMaxR >= 8 ? 8 : 1)>(e, len);
-    else wave_sort<kByRow>(e, len);
+    // the large-LDS kernels order columns of up to kRankMax entries by
+    // counting and come here for them only when a row run is longer than the
+    // in-register merge takes, so they need no register network (its code
+    // would only enlarge the kernels' instruction footprint)
+    wave_sort<kByRow>(e, len);
 }
 
 // ---- counting sorts for the kernels that run one wave per CU
