@@ -210,7 +210,7 @@ void launch_mul64(const int *in, int *out, int n, hipStream_t s);
 // s = t / G at off[c] + s * P * G + i * G + (t % G).  slices[c] =
 // ceil(width[c] / G); size[c] = slices[c] * P * G.
 constexpr int kSweepThreads = 256;
-constexpr int kTailThreads = 1024;          // 4 sweep groups
+constexpr int kTailThreads = 512;           // 8 waves: fewer idle waves per narrow level than 16
 __host__ __device__ inline int sweep_rows_pow2(int cnt)
 {
     return cnt <= 1 ? 1 : 1 << (32 - __builtin_clz((unsigned)(cnt - 1)));
@@ -238,7 +238,7 @@ void launch_sweep_layout(int nchunks, const int *cnt, const int *width, int *sli
 // are padding (index ntail, value 0); one more block of padding at off[ntl] is
 // what lanes without a slot load.  The workgroup's LDS holds 0 at value slot
 // ntail, so a padded contribution is -0 * 0 and needs no arithmetic mask.
-constexpr int kTailMaxRows = 1024;
+constexpr int kTailMaxRows = kTailThreads;
 constexpr int kTailBlock = 8;
 __host__ __device__ inline int tail_group(int cnt)
 {

@@ -491,7 +491,7 @@ __global__ void __launch_bounds__(kSweepThreads) k_sweep_bulk(SolveArgs a, const
 }
 
 // ---------------------------------------------------------------- tail
-// The narrow trailing levels (rows [tail_pos0, n)) run in one 1024-thread
+// The narrow trailing levels (rows [tail_pos0, n)) run in one kTailThreads
 // workgroup whose LDS holds the tail's solution values, so the level-to-level
 // dependency chain goes through LDS and a barrier, not through HBM.
 // Forward: k_tail_prep folds every tail row's entries in the wide levels

@@ -111,7 +111,7 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (lo && std::string(lo) == "groups") h->level_order = 3;
         const char *nm = std::getenv("RCHOL_NAP_MAX");          // polling back-off cap (1..64)
         if (nm) h->nap_max = std::max(1, std::min(64, std::atoi(nm)));
-        const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..1024)
+        const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..kTailMaxRows + 1)
         if (tr) h->tail_max_rows = std::max(1, std::min(kTailMaxRows + 1, std::atoi(tr)));
         const char *sc = std::getenv("RCHOL_LIST_SEG_CAP");     // ready-list segment capacity limit (>= 1)
         if (sc) h->seg_cap_limit = std::max(1, std::atoi(sc));
@@ -2521,7 +2521,7 @@ static int rc_autotune_impl(rc_handle_t h, int reps, rc_tune_stats *st)
     if (!h->factored || !h->rhs_set) return fail(h, RC_ERR_STATE, "factor and rc_set_rhs must precede rc_autotune");
     if (reps <= 0) reps = 5;
     RC_HIP(hipSetDevice(h->opt.device));
-    const int tails[] = {0, 64, 128, 256, 512, 1025};
+    const int tails[] = {0, 64, 128, 256, kTailMaxRows + 1};
     const int grids[] = {h->sweep_blocks_max, h->sweep_blocks_max / 2, h->sweep_blocks_max / 4};
     auto reassemble = [&](int tail) -> int {
         h->tail_max_rows = tail;
