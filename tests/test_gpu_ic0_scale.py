@@ -215,7 +215,16 @@ def test_benchmark_size_factor_and_pcg_equal_oracle(rc, orc, wl, which):
     # sequential solves; PCG minimises the A-norm error)
     d = x[perm] - xo
     assert np.sqrt(d @ orc.spmv(sys_, d)) <= (1e-8 if prec == 64 else 1e-5) * np.sqrt(xo @ orc.spmv(sys_, xo))
-    assert abs(so["rel_res_true"] - oo["rel_res_true"]) <= (0.05 if prec == 64 else 0.25) * oo["rel_res_true"]
+    # the true residual where the stop rule fires: with FP64 sweeps the two
+    # runs agree closely; with FP32 sweeps each row's sum is grouped
+    # differently from the oracle's sequential one (the grouping follows the
+    # sweep layout), the Krylov iterates drift apart by rounding, and the
+    # residual norm (unlike the A-norm error checked above) is not monotone
+    # in the iteration, so it is held to a factor of two
+    if prec == 64:
+        assert abs(so["rel_res_true"] - oo["rel_res_true"]) <= 0.05 * oo["rel_res_true"]
+    else:
+        assert 0.5 * oo["rel_res_true"] <= so["rel_res_true"] <= 2.0 * oo["rel_res_true"], (so, oo)
     s.close()
 
 
