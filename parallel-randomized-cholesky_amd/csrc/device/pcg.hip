@@ -631,6 +631,10 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
         // a wave with no slot of this level skips its LDS work (its loads of
         // the padding block were issued all the same, see fetch)
         if ((t & ~(kWave - 1)) < stride) {
+            // the row's own value is read with the gathers (it is written only
+            // at the end of the level), not after the sum: one LDS round trip
+            // less on the level's critical path; other lanes read the zero slot
+            const T own = v[lead ? q : nt];
 #pragma unroll
             for (int u = 0; u < kTailPref; ++u) acc = fma_t(-l[u], v[j[u]], acc);
             if (t < stride) {
@@ -644,7 +648,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
                     for (int u = 0; u < kTailBlock; ++u) acc = fma_t(-ll[u], v[jj[u]], acc);
                 }
             }
-            if (lead) acc += v[q];
+            if (lead) acc += own;
             acc = group_sum(acc, G);
             if (lead) v[q] = acc;
         }
