@@ -608,9 +608,14 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     // kTailPref index/value slices of level s + 2 are loaded while level s is
     // solved (two levels of latency cover one global round trip); the loop
     // is unrolled by three so that no register copy (and no wait) sits
-    // between the loads and their use.
-    auto fetch = [&](int s, int4 &d, int *j, T *l) {
-        d = tm[s < ntl ? (kFwd ? s : ntl - 1 - s) : ntl];
+    // between the loads and their use.  Level descriptors are read from LDS
+    // one fetch ahead (dn), so a fetch starts without an LDS round trip; the
+    // lane term (a.zero is 0) keeps the read a per-lane one, so the compiler
+    // does not make it uniform (and wait for it) at the read but at its use.
+    auto desc = [&](int s) { return tm[(s < ntl ? (kFwd ? s : ntl - 1 - s) : ntl) + (t & (kWave - 1)) * a.zero]; };
+    auto fetch = [&](const int4 &dsrc, int4 &d, int *j, T *l) {
+        d = make_int4(__builtin_amdgcn_readfirstlane(dsrc.x), __builtin_amdgcn_readfirstlane(dsrc.y),
+                      __builtin_amdgcn_readfirstlane(dsrc.z), __builtin_amdgcn_readfirstlane(dsrc.w));
         const int stride = sweep_rows_pow2(d.y) * tail_group(d.y);
         // every lane loads (the padding block when it has no slot), so the
         // loads are never predicated away, and nothing is computed from the
@@ -657,18 +662,23 @@ __global__ void __launch_bounds__(kTailThreads) k_tail_solve(SolveArgs a)
     int4 dA = make_int4(0, 0, 0, 0), dB = dA, dC = dA;
     int jA[kTailPref], jB[kTailPref], jC[kTailPref];
     T lA[kTailPref], lB[kTailPref], lC[kTailPref];
+    int4 dn = dA;
     if (ntl > 0) {
-        fetch(0, dA, jA, lA);
-        fetch(1, dB, jB, lB);
+        fetch(desc(0), dA, jA, lA);
+        fetch(desc(1), dB, jB, lB);
+        dn = desc(2);
     }
     // no exits inside the group: an exit path would carry the newest loads
     // to the loop header, where the compiler then waits for them every level
     for (int s = 0; s < ntl; s += 3) {
-        fetch(s + 2, dC, jC, lC);
+        fetch(dn, dC, jC, lC);
+        dn = desc(s + 3);
         solve(dA, jA, lA);
-        fetch(s + 3, dA, jA, lA);
+        fetch(dn, dA, jA, lA);
+        dn = desc(s + 4);
         solve(dB, jB, lB);
-        fetch(s + 4, dB, jB, lB);
+        fetch(dn, dB, jB, lB);
+        dn = desc(s + 5);
         solve(dC, jC, lC);
     }
     for (int q = t; q < nt; q += kTailThreads) {
