@@ -150,12 +150,21 @@ class Run:
             s.order(rc.ORDER_NNZ_SORT, 1)
         self.order_s = time.perf_counter() - t
         self.order_name = order
+        # within-level row order of the sweeps: the recursive bisection of the
+        # graph (computed once per matrix on the host, like the ordering)
+        self.locality = "input"
+        self.locality_s = 0.0
+        if not self.batch and args.locality == "bisection":
+            s.set_locality(rc.LOCALITY_BISECTION)
+            self.locality = "bisection"
+            self.locality_s = s.locality()[1]
         self.rhs_kind = rc.RHS_NORMAL if cfg["rhs"] == "normal" else rc.RHS_NORMAL_PROJECTED
         self.b = s.make_rhs(self.rhs_kind, 42)
         self.B = (np.stack([s.make_rhs(self.rhs_kind, 42 + q) for q in range(self.nrhs)], axis=1)
                   if self.nrhs > 1 else None)
         self.s = s
-        log(f"[bench] {name}: input {self.input_s:.1f}s, ordering ({order}) {self.order_s:.2f}s")
+        log(f"[bench] {name}: input {self.input_s:.1f}s, ordering ({order}) {self.order_s:.2f}s, "
+            f"locality ({self.locality}) {self.locality_s:.2f}s")
 
     def set_rhs(self):
         if self.B is not None:
@@ -441,6 +450,8 @@ def bench_north_star(rc, wl, args):
         res["equal_accuracy"] = run.equal_accuracy(min(2, args.steps))
         res["order_s"] = run.order_s
         res["input_s"] = run.input_s
+        res["locality"] = run.locality
+        res["locality_s"] = run.locality_s
         res["autotune"] = run.tune
         runs[key] = run
         out[key] = res
@@ -562,7 +573,7 @@ def bench_single(rc, wl, args, name):
                     res_t["mean_s"], "f64" if cfg["prec"] == 64 else "f32-factor/f64-pcg", cfg_line)
     res["roofline"] = roofline(cfg["workload"], kern, dom, kname)
     res_t.update({"kernels": kern, "autotune": run.tune, "error_vs_fp64": err64, "order_s": run.order_s,
-                  "input_s": run.input_s})
+                  "input_s": run.input_s, "locality": run.locality, "locality_s": run.locality_s})
     res["detail"] = res_t
     run.close()
     return res, cfg
@@ -580,6 +591,10 @@ def main():
     ap.add_argument("--no-thread-sweep", action="store_true")
     ap.add_argument("--cpu-max-n", type=int, default=2_200_000, help="rows of the CPU leg of the batched workloads")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--locality", default="bisection", choices=["input", "bisection"],
+                    help="within-level row order of the sweeps and SpMV tiles (bisection: recursive bisection "
+                         "of the graph, computed once per matrix on the host, outside the timed steps like the "
+                         "ordering; input: the input numbering)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the per-structure sweep launch tuning done once before the timed steps")
     args = ap.parse_args()

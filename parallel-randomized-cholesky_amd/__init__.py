@@ -27,6 +27,7 @@ ERR_TIMEOUT, ERR_BREAKDOWN, ERR_NO_DEVICE, ERR_NOMEM, ERR_NOT_SYMMETRIC, ERR_MAX
 
 INPUT_LAPLACIAN, INPUT_GROUNDED, INPUT_SDDM = 0, 1, 2
 ORDER_AS_IS, ORDER_NNZ_SORT, ORDER_RANDOM, ORDER_AMD, ORDER_LEARNED, ORDER_ND = 0, 1, 2, 3, 4, 5
+LOCALITY_INPUT, LOCALITY_BISECTION = 0, 1
 STOP_REFERENCE, STOP_TRUE_RESIDUAL = 0, 1
 RHS_ZERO_SUM_UNIFORM, RHS_NORMAL, RHS_NORMAL_PROJECTED = 0, 1, 2
 KERNEL_SPMV, KERNEL_TRSV_FWD, KERNEL_TRSV_BWD, KERNEL_PCG_ITER, KERNEL_COPY = 0, 1, 2, 3, 4
@@ -43,6 +44,7 @@ EXPORTS = (
     "rc_set_batch", "rc_get_batch", "rc_solve_batch", "rc_get_batch_stats", "rc_autotune",
     "rc_set_rhs_multi", "rc_solve_multi_resident", "rc_get_solution_multi", "rc_solve_noalloc",
     "rc_get_multi_stats", "rc_solve_group", "rc_set_solve_options", "rc_factor_ic0",
+    "rc_set_locality", "rc_get_locality",
 )
 
 
@@ -140,6 +142,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_write_prepared_mtx.argtypes = [vp, ctypes.c_char_p]
     lib.rc_solve_multi.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_double, P(SolveStats)]
     lib.rc_set_permutation.argtypes = [vp, vp]
+    lib.rc_set_locality.argtypes = [vp, ctypes.c_int]
+    lib.rc_get_locality.argtypes = [vp, vp, P(ctypes.c_double)]
     lib.rc_gnn_weight_count.argtypes = []
     lib.rc_gnn_weight_count.restype = ctypes.c_int64
     lib.rc_gnn_scores.argtypes = [vp, vp, vp]
@@ -282,6 +286,19 @@ class Solver:
     def set_permutation(self, perm):
         p = np.ascontiguousarray(perm, dtype=np.int32)
         self._check(self.lib.rc_set_permutation(self.h, _ptr(p)))
+
+    def set_locality(self, mode=LOCALITY_BISECTION):
+        """Within-level row order of the sweeps: input numbering or recursive
+        bisection of the graph (computed once per matrix on the host)."""
+        self._check(self.lib.rc_set_locality(self.h, mode))
+
+    def locality(self):
+        """(rank per input vertex, host seconds of the last bisection)."""
+        n, _ = self.size()
+        r = np.empty(n, dtype=np.int32)
+        sec = ctypes.c_double(0.0)
+        self._check(self.lib.rc_get_locality(self.h, _ptr(r), ctypes.byref(sec)))
+        return r, sec.value
 
     def gnn_scores(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)

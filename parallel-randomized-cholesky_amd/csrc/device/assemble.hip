@@ -701,6 +701,19 @@ void launch_gather_t(const T *src, const int *idx, T *dst, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gather_t<T>, dim3(2048), dim3(256), 0, s, src, idx, dst, n);
 }
+// keys[p] = locality rank of factored row p's input vertex (rows without one,
+// such as an appended ground vertex, after all of them)
+__global__ void k_locality_keys(int ns, const int *perm, int nuser, const int *rank, uint32_t *keys)
+{
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < ns; p += gridDim.x * blockDim.x) {
+        const int u = perm[p];
+        keys[p] = (uint32_t)((u >= 0 && u < nuser) ? rank[u] : nuser);
+    }
+}
+void launch_locality_keys(int ns, const int *perm, int nuser, const int *rank, uint32_t *keys, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_locality_keys, dim3(2048), dim3(256), 0, s, ns, perm, nuser, rank, keys);
+}
 void launch_gather_i(const int *src, const int *idx, int *dst, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gather_t<int>, dim3(2048), dim3(256), 0, s, src, idx, dst, n);

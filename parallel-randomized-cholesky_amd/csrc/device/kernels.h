@@ -180,6 +180,7 @@ void launch_pack_meta(int nchunks, const int *start, const int *cnt, const int *
                       hipStream_t s);
 template <typename T> void launch_gather_t(const T *src, const int *idx, T *dst, int n, hipStream_t s);
 void launch_gather_i(const int *src, const int *idx, int *dst, int n, hipStream_t s);
+void launch_locality_keys(int ns, const int *perm, int nuser, const int *rank, uint32_t *keys, hipStream_t s);
 
 // Sliced ELL (64 rows per slice, entry t of slice row l at off[c] + 64 t + l).
 // Slice c covers rows order[start[c] + l], l < cnt[c] (chunk mode) or

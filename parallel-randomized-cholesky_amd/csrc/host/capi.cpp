@@ -153,6 +153,7 @@ static void reset_system(rc_handle_t h)
     h->factored = false;
     h->rhs_set = false;
     h->perm.clear();
+    h->loc_rank.clear();
     h->xref.clear();
     h->nsys = 0;
     h->sys_off.clear();
@@ -273,6 +274,43 @@ static int rc_get_size_impl(rc_handle_t h, int64_t *n, int64_t *nnz)
 static int gnn_setup(rc_handle_t h, bool keep);
 static GnnArgs gnn_args(rc_handle_t h);
 
+// The recursive bisection order of the loaded graph as ranks per user vertex
+// (once per matrix; the device copy is made at preparation).
+static void compute_locality(rc_handle_t h)
+{
+    h->loc_rank.clear();
+    if (h->locality != RC_LOCALITY_BISECTION || h->kind < 0 || h->n_user <= 0) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::vector<int> p = rb_perm(h->base, h->n_user, 1, h->opt.host_threads);
+    h->loc_rank.assign((size_t)h->n_user, 0);
+    for (int i = 0; i < h->n_user; ++i) h->loc_rank[(size_t)p[(size_t)i]] = i;
+    h->locality_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static int rc_set_locality_impl(rc_handle_t h, int mode)
+{
+    RC_CHECK_HANDLE();
+    if (mode != RC_LOCALITY_INPUT && mode != RC_LOCALITY_BISECTION) return fail(h, RC_ERR_ARG, "unknown locality mode");
+    if (mode == h->locality && (mode == RC_LOCALITY_INPUT || !h->loc_rank.empty() || h->kind < 0)) return RC_OK;
+    h->locality = mode;
+    compute_locality(h);
+    // the level order is built at preparation/assembly: prepare again
+    h->prepared = false;
+    h->factored = false;
+    h->rhs_set = false;
+    invalidate_graph(h);
+    return RC_OK;
+}
+
+static int rc_get_locality_impl(rc_handle_t h, int *rank, double *seconds)
+{
+    RC_CHECK_HANDLE();
+    if (h->kind < 0 || !rank) return fail(h, RC_ERR_ARG, "no matrix or null array");
+    for (int i = 0; i < h->n_user; ++i) rank[i] = h->loc_rank.empty() ? i : h->loc_rank[(size_t)i];
+    if (seconds) *seconds = h->locality_s;
+    return RC_OK;
+}
+
 static int rc_order_impl(rc_handle_t h, int method, uint64_t seed)
 {
     RC_CHECK_HANDLE();
@@ -311,6 +349,7 @@ static int rc_order_impl(rc_handle_t h, int method, uint64_t seed)
     default: return fail(h, RC_ERR_ARG, "unknown ordering");
     }
     h->perm = std::move(p);
+    if (h->locality == RC_LOCALITY_BISECTION && h->loc_rank.empty()) compute_locality(h);
     h->host_ready = false;
     h->prepared = false;
     h->factored = false;
@@ -348,6 +387,7 @@ static int rc_set_permutation_impl(rc_handle_t h, const int *perm)
         seen[(size_t)perm[i]] = 1;
     }
     h->perm.assign(perm, perm + nv);
+    if (h->locality == RC_LOCALITY_BISECTION && h->loc_rank.empty()) compute_locality(h);
     h->host_ready = false;
     h->prepared = false;
     h->factored = false;
@@ -889,7 +929,15 @@ static int assemble(rc_handle_t h, double *ms_out)
     } else if (h->level_order == 1) {
         // rows of a level kept in the loaded (user) order: neighbouring rows of
         // a chunk are neighbours in the input numbering, so gathers share lines
-        launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        if (!h->loc_rank.empty()) {
+            // rows in the recursive bisection order of the input graph
+            launch_locality_keys(ns, h->perm_d.as<int>(), h->n_user, h->loc_d.as<int>(), k0, s);
+            launch_iota(h->order.as<int>(), ns, s);
+            radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, bit_width(h->n_user),
+                             h->tmp_b.as<int>(), s);
+        } else {
+            launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        }
         launch_gather_i(h->level.as<int>(), h->order.as<int>(), (int *)k0, ns, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits, h->tmp_b.as<int>(), s);
         launch_level_keys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, k1, s);
@@ -1123,12 +1171,20 @@ static int assemble(rc_handle_t h, double *ms_out)
     // the SpMV visits the tiles in the user order of their first rows: a row's
     // neighbours sit in other levels, i.e. far apart in positions, but near it
     // in the input numbering, so tiles visited together gather from the same
-    // lines (grids: a few neighbouring planes; batches: one system at a time)
+    // lines (grids: a few neighbouring planes; batches: one system at a time).
+    // With the bisection order inside levels a tile is a compact region, and
+    // the tiles are visited in the bisection order of their first rows.
     RC_HIP(h->tile_ord.ensure(tsz * sizeof(int)));
     RC_HIP(h->tile_k.ensure(2 * tsz * sizeof(uint32_t)));
     RC_HIP(h->tile_v.ensure(tsz * sizeof(int)));
     RC_HIP(h->tile_tmp.ensure(radix_tmp_ints((int)tsz) * sizeof(int)));
-    launch_tile_keys(ntiles, h->idx_user.as<int>(), h->tile_k.as<uint32_t>(), h->tile_ord.as<int>(), s);
+    const int *tile_src = h->idx_user.as<int>();
+    if (!h->loc_rank.empty() && h->nsys == 0) {
+        RC_HIP(h->loc_pos.ensure(nsz * sizeof(int)));
+        launch_locality_keys(ns, h->idx_user.as<int>(), h->n_user, h->loc_d.as<int>(), h->loc_pos.as<uint32_t>(), s);
+        tile_src = h->loc_pos.as<int>();
+    }
+    launch_tile_keys(ntiles, tile_src, h->tile_k.as<uint32_t>(), h->tile_ord.as<int>(), s);
     radix_sort_pairs(h->tile_k.as<uint32_t>(), h->tile_ord.as<int>(), h->tile_k.as<uint32_t>() + tsz,
                      h->tile_v.as<int>(), ntiles, std::max(1, bit_width(ns)), h->tile_tmp.as<int>(), s);
     if (h->nsys > 0) {
@@ -2615,6 +2671,16 @@ int rc_load_mtx(rc_handle_t h, const char *path, int kind)
 int rc_get_size(rc_handle_t h, int64_t *n, int64_t *nnz)
 {
     return guarded(h, [&]() { return rc_get_size_impl(h, n, nnz); });
+}
+
+int rc_set_locality(rc_handle_t h, int mode)
+{
+    return guarded(h, [&]() { return rc_set_locality_impl(h, mode); });
+}
+
+int rc_get_locality(rc_handle_t h, int *rank, double *seconds)
+{
+    return guarded(h, [&]() { return rc_get_locality_impl(h, rank, seconds); });
 }
 
 int rc_order(rc_handle_t h, int method, uint64_t seed)

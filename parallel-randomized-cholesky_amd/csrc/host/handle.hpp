@@ -70,6 +70,9 @@ struct rc_handle_s {
     Csc base;
     int n_user = 0;
     std::vector<int> perm;          // perm[new] = old (user numbering)
+    int locality = 0;               // RC_LOCALITY_*: within-level row order of the sweeps
+    std::vector<int> loc_rank;      // rank[user] in the recursive bisection order (empty: input order)
+    double locality_s = 0.0;        // host seconds of the last bisection
     std::vector<float> gnn_w;
 
     // prepared system
@@ -133,7 +136,7 @@ struct rc_handle_s {
 
     // PCG
     DevVec a_ptr, a_ind, a_val;
-    DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d;
+    DevVec x, r, ap, p0, p1, y, z, b, bu, scal, partials, ctrs, perm_d, loc_d, loc_pos;
     DevVec calib;                   // copy-kernel calibration buffer (rc_time_kernel)
     int reduce_blocks = 1024;
     int sweep_blocks = 1024;

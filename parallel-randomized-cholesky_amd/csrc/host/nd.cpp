@@ -15,6 +15,12 @@
 // (no separator).  Leaves and separators are ordered by ascending degree, ties
 // in a seeded random order.  Subgraphs are disjoint, so independent ones are
 // processed by several host threads at once over shared per-vertex arrays.
+//
+// rb_perm is the same recursion with each separator placed between its two
+// halves instead of after them: a recursive bisection order, in which the
+// vertices of every subgraph (a compact region of the graph) are contiguous.
+// The triangular sweeps order the rows of each level by it (locality of their
+// dependency gathers).
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
@@ -46,6 +52,7 @@ struct NdState {
     std::vector<int> perm;         // perm[new] = old
     std::vector<uint32_t> key;     // seeded random tie-break per vertex
     std::atomic<int> next_id{1};
+    bool sep_last = true;          // false: separator between the halves (bisection order)
 };
 
 // Owner of vertex w.  Neighbours of a subgraph's vertices may belong to a
@@ -163,20 +170,22 @@ void split(NdState &S, NdTask &t, std::vector<NdTask> &kids)
             else A.push_back(v);
         }
     }
-    sort_by_degree(S, Sep);
-    const int64_t sa = (int64_t)A.size(), sb = (int64_t)B.size();
-    std::copy(Sep.begin(), Sep.end(), S.perm.begin() + t.out + sa + sb);
+    if (S.sep_last) sort_by_degree(S, Sep);     // the bisection order keeps the separator in BFS order (locality)
+    const int64_t sa = (int64_t)A.size(), sb = (int64_t)B.size(), ss = (int64_t)Sep.size();
+    const int64_t sep_at = S.sep_last ? t.out + sa + sb : t.out + sa;
+    std::copy(Sep.begin(), Sep.end(), S.perm.begin() + sep_at);
     if (sa) kids.push_back({std::move(A), t.out});
-    if (sb) kids.push_back({std::move(B), t.out + sa});
+    if (sb) kids.push_back({std::move(B), S.sep_last ? t.out + sa : t.out + sa + ss});
 }
 
 }  // namespace
 
-std::vector<int> nd_perm(const Csc &a, int nvert, uint64_t seed, int threads)
+static std::vector<int> dissect(const Csc &a, int nvert, uint64_t seed, int threads, bool sep_last)
 {
     NdState S;
     S.a = &a;
     S.nvert = nvert;
+    S.sep_last = sep_last;
     S.deg.assign((size_t)nvert, 0);
     for (int v = 0; v < nvert; ++v)
         for (int64_t p = a.ptr[(size_t)v]; p < a.ptr[(size_t)v + 1]; ++p) {
@@ -239,6 +248,16 @@ std::vector<int> nd_perm(const Csc &a, int nvert, uint64_t seed, int threads)
         for (auto &x : th) x.join();
     }
     return std::move(S.perm);
+}
+
+std::vector<int> nd_perm(const Csc &a, int nvert, uint64_t seed, int threads)
+{
+    return dissect(a, nvert, seed, threads, true);
+}
+
+std::vector<int> rb_perm(const Csc &a, int nvert, uint64_t seed, int threads)
+{
+    return dissect(a, nvert, seed, threads, false);
 }
 
 }  // namespace rc

@@ -182,6 +182,10 @@ int prepare_device(rc_handle_t h)
     const int ns = h->ns;
     RC_HIP(h->perm_d.ensure((size_t)std::max(ns, 1) * sizeof(int)));
     RC_HIP(hipMemcpyAsync(h->perm_d.p, h->perm.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!h->loc_rank.empty()) {
+        RC_HIP(h->loc_d.ensure((size_t)h->n_user * sizeof(int)));
+        RC_HIP(hipMemcpyAsync(h->loc_d.p, h->loc_rank.data(), (size_t)h->n_user * sizeof(int), hipMemcpyHostToDevice, s));
+    }
     RC_HIP(hipGetLastError());
     RC_HIP(hipStreamSynchronize(s));
     return RC_OK;

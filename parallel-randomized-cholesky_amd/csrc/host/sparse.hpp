@@ -49,6 +49,8 @@ std::vector<int> amd_perm(const Csc &a, int nvert);
 // Nested dissection by BFS level-structure separators (nd.cpp) on the first
 // nvert vertices (ties broken by `seed`), `threads` host threads (0 = all).
 std::vector<int> nd_perm(const Csc &a, int nvert, uint64_t seed, int threads);
+// Recursive bisection order (nd_perm with each separator between its halves).
+std::vector<int> rb_perm(const Csc &a, int nvert, uint64_t seed, int threads);
 
 // Strict lower part as int32 CSC with |values| (cpu pre_process.hpp:763-850).
 int lower_part(const Csc &a, std::vector<int> &ptr, std::vector<int> &row, std::vector<double> &w);

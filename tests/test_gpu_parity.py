@@ -261,6 +261,33 @@ def test_level_orders_agree_with_oracle(rc, orc, wl, order, prec, monkeypatch):
     assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("name", ["rgg_3000", "grid3d_20_log", "sddm_7"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_bisection_locality_agrees_with_oracle(rc, orc, wl, name, prec):
+    """Rows of each level in the recursive bisection order of the graph
+    (rc_set_locality): the factor is the oracle's bit for bit (it does not
+    depend on the sweeps' layout) and PCG agrees with the oracle's."""
+    say(f"[gpu] bisection locality case={name} prec={prec}")
+    s, data, kind = _solver(rc, wl, name, prec)
+    s.set_locality(rc.LOCALITY_BISECTION)
+    s.factor()
+    rp, ci, va, dg = s.get_factor()
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    mask = np.ones(len(ci), bool)
+    mask[rp[:-1]] = False
+    np.testing.assert_array_equal(ci[mask], f.ind)
+    if prec == 64:
+        np.testing.assert_array_equal(va[mask], f.val)
+    else:
+        np.testing.assert_array_equal(va[mask], f.val.astype(np.float32).astype(np.float64))
+    b = s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42)
+    x, st = s.solve(b, tol=1e-6)
+    xo, so = orc.pcg(sys_, f, b[perm], tol=1e-6, precision=prec)
+    assert abs(st["iterations"] - so["iterations"]) <= 1
+    tol = 1e-7 if prec == 64 else 1e-4
+    assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("gmin", ["1", "2", "4"])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_sweep_chunk_sizes_agree_with_oracle(rc, orc, wl, gmin, prec, monkeypatch):

@@ -215,3 +215,29 @@ def test_cli_orders_and_writes_the_prepared_matrix(rc, tmp_path):
     assert r.returncode == 0, r.stderr
     assert json.loads(r.stdout.strip().splitlines()[-1])["order"] == rc.ORDER_ND
     assert open(out).readline().startswith("%%MatrixMarket")
+
+
+@pytest.mark.parametrize("dims", [(48, 48), (14, 14, 14)])
+def test_bisection_locality_order(rc, wl, dims):
+    """The sweeps' within-level order: a permutation of the input vertices
+    that depends on the graph only (not on the elimination ordering nor on
+    the host thread count), with the input numbering when switched off."""
+    n, r, c, v = wl.grid_laplacian(dims)
+    s = _host(rc)
+    s.set_triplets(n, r, c, v)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    rank0, _ = s.locality()
+    np.testing.assert_array_equal(rank0, np.arange(n))              # off: input numbering
+    s.set_locality(rc.LOCALITY_BISECTION)
+    rank, secs = s.locality()
+    assert sorted(rank.tolist()) == list(range(n)) and secs >= 0.0
+    assert not np.array_equal(rank, np.arange(n))
+    s.order(rc.ORDER_RANDOM, 7)                                         # another elimination ordering
+    np.testing.assert_array_equal(s.locality()[0], rank)
+    t = _host(rc, 1)
+    t.set_locality(rc.LOCALITY_BISECTION)                               # set before the matrix: computed at order
+    t.set_triplets(n, r, c, v)
+    t.order(rc.ORDER_NNZ_SORT, 1)
+    np.testing.assert_array_equal(t.locality()[0], rank)                # one host thread, same order
+    s.set_locality(rc.LOCALITY_INPUT)
+    np.testing.assert_array_equal(s.locality()[0], np.arange(n))
