@@ -288,6 +288,24 @@ def test_bisection_locality_agrees_with_oracle(rc, orc, wl, name, prec):
     assert np.linalg.norm(x[perm] - xo) <= tol * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_bisection_locality_multi_rhs_agrees_with_oracle(rc, orc, wl, prec):
+    """Several right-hand sides on one factor with the bisection row order:
+    every column agrees with the oracle's PCG of that right-hand side."""
+    say(f"[gpu] bisection locality multi-RHS case=grid3d_20_log prec={prec}")
+    s, data, kind = _solver(rc, wl, "grid3d_20_log", prec)
+    s.set_locality(rc.LOCALITY_BISECTION)
+    s.factor()
+    _, f, sys_, perm = _oracle(orc, data, kind)
+    B = np.stack([s.make_rhs(rc.RHS_NORMAL_PROJECTED, 42 + q) for q in range(3)], axis=1)
+    X, sts = s.solve_multi(B, tol=1e-6)
+    tol = 1e-7 if prec == 64 else 1e-4
+    for q in range(3):
+        xo, so = orc.pcg(sys_, f, B[perm, q], tol=1e-6, precision=prec)
+        assert abs(sts[q]["iterations"] - so["iterations"]) <= 1
+        assert np.linalg.norm(X[perm, q] - xo) <= tol * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("gmin", ["1", "2", "4"])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_sweep_chunk_sizes_agree_with_oracle(rc, orc, wl, gmin, prec, monkeypatch):
