@@ -151,7 +151,9 @@ class Run:
         self.order_s = time.perf_counter() - t
         self.order_name = order
         # within-level row order of the sweeps: the recursive bisection of the
-        # graph (computed once per matrix on the host, like the ordering)
+        # graph (computed once per matrix on the host, like the ordering); a
+        # batch keeps the input order (measured neutral: every system's rows
+        # fit in the L2, and the per-system bisections take seconds)
         self.locality = "input"
         self.locality_s = 0.0
         if not self.batch and args.locality == "bisection":

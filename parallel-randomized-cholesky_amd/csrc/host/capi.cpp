@@ -281,9 +281,31 @@ static void compute_locality(rc_handle_t h)
     h->loc_rank.clear();
     if (h->locality != RC_LOCALITY_BISECTION || h->kind < 0 || h->n_user <= 0) return;
     const auto t0 = std::chrono::steady_clock::now();
-    const std::vector<int> p = rb_perm(h->base, h->n_user, 1, h->opt.host_threads);
     h->loc_rank.assign((size_t)h->n_user, 0);
-    for (int i = 0; i < h->n_user; ++i) h->loc_rank[(size_t)p[(size_t)i]] = i;
+    if (h->nsys == 0) {
+        const std::vector<int> p = rb_perm(h->base, h->n_user, 1, h->opt.host_threads);
+        for (int i = 0; i < h->n_user; ++i) h->loc_rank[(size_t)p[(size_t)i]] = i;
+    } else {
+        // a batch: each system bisected on its own, ranks kept inside the
+        // system's row range (the systems stay contiguous and in order)
+        for (int q = 0; q < h->nsys; ++q) {
+            const int64_t o = h->sys_off[(size_t)q], e = h->sys_off[(size_t)q + 1];
+            Csc sub;
+            sub.n = (int)(e - o);
+            sub.ptr.assign((size_t)sub.n + 1, 0);
+            for (int64_t j = o; j < e; ++j) {
+                for (int64_t k = h->base.ptr[(size_t)j]; k < h->base.ptr[(size_t)j + 1]; ++k) {
+                    const int i = h->base.ind[(size_t)k];
+                    if (i < o || i >= e) continue;
+                    sub.ind.push_back((int)(i - o));
+                    sub.val.push_back(h->base.val[(size_t)k]);
+                }
+                sub.ptr[(size_t)(j - o) + 1] = (int64_t)sub.ind.size();
+            }
+            const std::vector<int> p = rb_perm(sub, sub.n, 1, h->opt.host_threads);
+            for (int i = 0; i < sub.n; ++i) h->loc_rank[(size_t)(o + p[(size_t)i])] = (int)(o + i);
+        }
+    }
     h->locality_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1179,7 +1201,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     RC_HIP(h->tile_v.ensure(tsz * sizeof(int)));
     RC_HIP(h->tile_tmp.ensure(radix_tmp_ints((int)tsz) * sizeof(int)));
     const int *tile_src = h->idx_user.as<int>();
-    if (!h->loc_rank.empty() && h->nsys == 0) {
+    if (!h->loc_rank.empty()) {
         RC_HIP(h->loc_pos.ensure(nsz * sizeof(int)));
         launch_locality_keys(ns, h->idx_user.as<int>(), h->n_user, h->loc_d.as<int>(), h->loc_pos.as<uint32_t>(), s);
         tile_src = h->loc_pos.as<int>();

@@ -418,16 +418,19 @@ def _block_diag(systems):
     return off, np.concatenate(R), np.concatenate(C), np.concatenate(V)
 
 
-@pytest.mark.parametrize("many", [False, True])
+@pytest.mark.parametrize("many,loc", [(False, False), (True, False), (False, True)])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_batch_matches_oracle_per_system(rc, orc, wl, prec, many):
+def test_batch_matches_oracle_per_system(rc, orc, wl, prec, many, loc):
     """Every system of a batch runs its own PCG: each system's iterate equals
     the oracle PCG on the block-diagonal system with the other systems'
-    right-hand sides zeroed (which is that system's PCG alone)."""
-    say(f"[gpu] batch prec={prec} many={many}")
+    right-hand sides zeroed (which is that system's PCG alone); also with the
+    rows of each level in every system's bisection order (loc)."""
+    say(f"[gpu] batch prec={prec} many={many} loc={loc}")
     systems = _batch_systems(wl, many)
     s = rc.Solver(seed=11, precision=prec)
     s.set_batch(systems)
+    if loc:
+        s.set_locality(rc.LOCALITY_BISECTION)
     s.order(rc.ORDER_NNZ_SORT, 1)
     fs = s.factor()
     k, off = s.batch()

@@ -241,3 +241,24 @@ def test_bisection_locality_order(rc, wl, dims):
     np.testing.assert_array_equal(t.locality()[0], rank)                # one host thread, same order
     s.set_locality(rc.LOCALITY_INPUT)
     np.testing.assert_array_equal(s.locality()[0], np.arange(n))
+
+
+def test_bisection_locality_order_batch(rc, wl):
+    """A batch: every system is bisected on its own; its ranks stay inside its
+    row range and equal the ranks of the same system loaded alone."""
+    systems = [wl.random_geometric_graph(k, 5 + k) for k in (300, 1000, 450)]
+    s = _host(rc)
+    s.set_batch(systems)
+    s.set_locality(rc.LOCALITY_BISECTION)
+    s.order(rc.ORDER_NNZ_SORT, 1)
+    rank, _ = s.locality()
+    off = 0
+    for n, r, c, v in systems:
+        part = rank[off:off + n]
+        assert sorted(part.tolist()) == list(range(off, off + n))
+        t = _host(rc)
+        t.set_triplets(n, r, c, v)
+        t.set_locality(rc.LOCALITY_BISECTION)
+        t.order(rc.ORDER_NNZ_SORT, 1)
+        np.testing.assert_array_equal(part - off, t.locality()[0])
+        off += n
