@@ -109,6 +109,7 @@ static int rc_create_impl(rc_handle_t *out, const rc_options *opt)
         if (lo && std::string(lo) == "spatial") h->level_order = 1;
         if (lo && std::string(lo) == "bucket") h->level_order = 2;
         if (lo && std::string(lo) == "groups") h->level_order = 3;
+        if (lo) h->level_order_set = true;
         const char *nm = std::getenv("RCHOL_NAP_MAX");          // polling back-off cap (1..64)
         if (nm) h->nap_max = std::max(1, std::min(64, std::atoi(nm)));
         const char *tr = std::getenv("RCHOL_TAIL_ROWS");        // tail level width limit (1..kTailMaxRows + 1)
@@ -922,23 +923,37 @@ static int assemble(rc_handle_t h, double *ms_out)
     int len_bits = std::max(0, std::min(16, 32 - lvl_bits - sys_bits));
     RC_HIP(h->order.ensure(nsz * sizeof(int)));
     RC_HIP(h->pos_of.ensure(nsz * sizeof(int)));
+    // rows of a level: in the input order (spatial) or, under the bisection
+    // locality order and for systems of more than 4 M rows, in width classes
+    // each in that order (measured: 256^3 sweeps 2.93 / 2.59 -> 2.80 / 2.52 ms;
+    // at 128^3 the classes cost more than they gain, 0.77 -> 0.84 ms forward)
+    const bool classes_by_rank = h->level_order == 1 && !h->loc_rank.empty() && !h->level_order_set &&
+                                 h->nsys == 0 && ns > (1 << 22);
+    const int level_order = classes_by_rank ? 3 : h->level_order;
     if (h->nsys > 0) {
         launch_level_keys_sys(ns, h->level.as<int>(), h->l_ptr.as<int>(), len_bits, h->sys_fac.as<int>(), sys_bits, k0,
                               s);
         launch_iota(h->order.as<int>(), ns, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + sys_bits + len_bits,
                          h->tmp_b.as<int>(), s);
-    } else if (h->level_order == 3 && lvl_bits + 2 <= 32) {
+    } else if (level_order == 3 && lvl_bits + 2 <= 32) {
         // rows of a level in four classes by the lanes per row their width
         // needs, each class in the loaded (user) order: short rows get their
         // own (wider) chunks instead of the chunk size of the level's widest
         // row, and neighbouring rows of a chunk stay neighbours in the input
-        launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        if (!h->loc_rank.empty()) {      // order = rows by locality rank
+            launch_locality_keys(ns, h->perm_d.as<int>(), h->n_user, h->loc_d.as<int>(), k0, s);
+            launch_iota(h->order.as<int>(), ns, s);
+            radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, bit_width(h->n_user),
+                             h->tmp_b.as<int>(), s);
+        } else {
+            launch_inverse_perm(ns, h->perm_d.as<int>(), h->order.as<int>(), s);    // order = rows by user index
+        }
         launch_level_group_keys(ns, h->order.as<int>(), h->level.as<int>(), h->l_ptr.as<int>(), h->lt_ptr.as<int>(),
                                 k0, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + 2, h->tmp_b.as<int>(), s);
         len_bits = 2;
-    } else if (h->level_order == 2 && lvl_bits + 3 <= 32) {
+    } else if (level_order == 2 && lvl_bits + 3 <= 32) {
         // rows of a level in classes of similar width (log2 of the wider of
         // the L and L^T rows, widest first), each class in the loaded (user)
         // order: chunks stay near-uniform in width and neighbouring rows of a
@@ -948,7 +963,7 @@ static int assemble(rc_handle_t h, double *ms_out)
                                  h->lt_ptr.as<int>(), k0, s);
         radix_sort_pairs(k0, h->order.as<int>(), k1, h->pos_of.as<int>(), ns, lvl_bits + 3, h->tmp_b.as<int>(), s);
         len_bits = 3;
-    } else if (h->level_order == 1) {
+    } else if (level_order == 1) {
         // rows of a level kept in the loaded (user) order: neighbouring rows of
         // a chunk are neighbours in the input numbering, so gathers share lines
         if (!h->loc_rank.empty()) {
@@ -978,7 +993,7 @@ static int assemble(rc_handle_t h, double *ms_out)
     // chunk segments: the levels, or, in the width-class order, every
     // (level, class) run of a level (8 per level), so each class gets the
     // chunk size of its own widest row rather than of the level's
-    const bool classes = sys_bits == 0 && ((h->level_order == 2 && len_bits == 3) || (h->level_order == 3 && len_bits == 2));
+    const bool classes = sys_bits == 0 && ((level_order == 2 && len_bits == 3) || (level_order == 3 && len_bits == 2));
     const int per_level = classes ? 1 << len_bits : 1;
     const int nseg = nl * per_level;
     const int seg_shift = classes ? 0 : len_bits + sys_bits;

@@ -123,6 +123,7 @@ struct rc_handle_s {
     int64_t nnz_lt = 0, ell_l = 0, ell_u = 0, ell_a = 0;
     int nlevels = 0, nchunks = 0, ntiles = 0;
     int tail_level = 0, bulk_chunks = 0;
+    bool level_order_set = false;   // RCHOL_LEVEL_ORDER given
     int level_order = 1;            // 0 length, 1 spatial (default), 2 spatial within length buckets,
                                     // 3 spatial within sweep-group classes
     // tail of the level order (LDS-resident solve)
